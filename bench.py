@@ -1,0 +1,225 @@
+"""Benchmark: 2TDVP site-updates/s at chi=128 on one MI355X, mixed vs FP64.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (the 41-cell single-instance 2TDVP case): N=41 cells, distance 2,
+activation interval [2,4] closed (Python range(2,5)), cells 19..21 alive,
+dt=0.1, chi_max=128, SVD cutoff 1e-10.  Two device-resident engines evolve the
+same state, one with complex64 contractions on FP32 matrix cores (mixed) and
+one in complex128 (plain); W untimed warmup steps grow the bonds, then EXACTLY
+K time steps of each engine are timed between synchronisations.
+
+A site-update is one two-site block update (2(N-1) per time step).  ``value``
+is the mixed engine's site-updates/s; the FP64 rate, their ratio and the
+largest |<n_i>_mixed - <n_i>_fp64| over the timed steps are reported beside it.
+
+``roofline``: the dominant contraction, the two-site effective-Hamiltonian
+application (GEMM over the left environment -> sparse MPO mixing -> GEMM over
+the right environment) at the bulk shape chi_l = chi_r = 128, D = 13, timed
+with HIP events on its stream; achieved = its algorithmic flop count
+8[4 D chi^2 (2 chi) + 16 D^2 chi^2] / average duration.
+
+``cpu_baseline``: the oracle (NumPy complex128, matrix-free Lanczos) running
+2TDVP site-updates on the same chi=128 state on the host cores for a bounded
+time; it is a reported baseline, not the target.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "quantum-cellular-automaton_amd")
+for p in (ROOT, PKG):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+N, DIST, IV, DT, CHI, EPS = 41, 2, range(2, 5), 0.1, 128, 1e-10
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def initial_state():
+    import states
+    return states.three_adjacent(N)
+
+
+def mpo():
+    from parameters import Rules
+    from tensor_networks import MPO
+    return MPO.hamiltonian_from_rules(Rules(N, IV, DIST, False))
+
+
+def heff2_roofline(torch, prec: int, reps: int = 20):
+    """Time the 2-site H_eff application at chi=128, D=13 with HIP events."""
+    import qca_native as qn
+    from tensor_networks import MPO
+    W = mpo().W[N // 2]
+    W2 = MPO.merge_mpo_tensor_pair(W, W)
+    D = W.shape[2]
+    chi = CHI
+    rng = np.random.default_rng(0)
+    ct = torch.complex128 if prec == 0 else torch.complex64
+
+    def rnd(*shape):
+        return torch.from_numpy((rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / np.sqrt(2)).to(ct).cuda()
+
+    x, L, R = rnd(4, chi, chi), rnd(chi, D, chi), rnd(chi, D, chi)
+    y = torch.empty_like(x)
+    Wh = np.ascontiguousarray(W2, dtype=np.complex128)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    for _ in range(3):
+        qn.call("qca_heff_apply", prec, 0, qn.ptr(x), qn.ptr(y), qn.ptr(L), qn.ptr(R), Wh.ctypes.data,
+                D, D, chi, chi, sp)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    # qca_heff_apply synchronises its stream after each call; time each call
+    # between events recorded on that same stream and average.
+    total = 0.0
+    for _ in range(reps):
+        e0.record(stream)
+        qn.call("qca_heff_apply", prec, 0, qn.ptr(x), qn.ptr(y), qn.ptr(L), qn.ptr(R), Wh.ctypes.data,
+                D, D, chi, chi, sp)
+        e1.record(stream)
+        e1.synchronize()
+        total += e0.elapsed_time(e1)
+    ms = total / reps
+    flops = 8.0 * (4.0 * D * chi * chi * (2 * chi) + 16.0 * D * D * chi * chi)
+    return ms, flops
+
+
+def cpu_baseline(As, seconds: float):
+    """Oracle 2TDVP site-updates on the host at the benchmark state (bounded)."""
+    from oracle import automaton as AU, tdvp as OT
+    cores = len(os.sched_getaffinity(0))
+    Ws = AU.mpo_tensors(N, DIST, IV)
+    o = OT.TDVPOracle([a.copy() for a in As], Ws, "2tdvp", DT, CHI, EPS, gauge="consistent",
+                      expm="lanczos")
+    h = DT / 2
+    done = 0
+    t0 = time.perf_counter()
+    # right half-sweep bond by bond (reference tdvp.py:107-125) until the budget is spent
+    for site in range(N - 1):
+        u, s, v = o._evolve_split(site, h)
+        o.A[site] = u
+        o.A[site + 1] = s[None, :, None] * v
+        if site < N - 2:
+            o._update_env("left", site)
+            o.A[site + 1] = o._evolve_site(site + 1, -h)
+        done += 1
+        if time.perf_counter() - t0 > seconds:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "site-updates/s", "cores": cores, "kind": "port",
+            "sample": f"oracle (NumPy complex128, matrix-free Lanczos) 2TDVP right half-sweep from the "
+                      f"warmed chi<=128 state: {done} site-updates in {dt:.1f} s, "
+                      f"BLAS threads = {os.environ.get('OPENBLAS_NUM_THREADS', 'default')}"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: every box in this pool has a single MI355X, and the "
+              "2TDVP sweep of one instance does not shard; run with --gpus 1", file=sys.stderr)
+        return 2
+    import torch
+    import qca_native as qn
+    if not torch.cuda.is_available():
+        print("no HIP device visible", file=sys.stderr)
+        return 2
+    Ws = mpo().W
+    psi = initial_state()
+    engines = {}
+    for name, prec in (("mixed", 1), ("fp64", 0)):
+        engines[name] = qn.Engine(Ws, psi.A, algorithm=2, precision=prec, chi_max=CHI, dt=DT, eps=EPS,
+                                  krylov_tol=1e-12 if prec else 1e-14)
+    log(f"engines built; warmup {a.warmup} steps")
+    maxdiff = 0.0
+    for w in range(a.warmup):
+        for e in engines.values():
+            e.step(1)
+        d = np.abs(engines["mixed"].rho()[:, 1, 1].real - engines["fp64"].rho()[:, 1, 1].real).max()
+        maxdiff = max(maxdiff, d)
+        if w % 5 == 4 or w == a.warmup - 1:
+            log(f"warmup {w + 1}: max bond {engines['fp64'].bond_dims().max()}, "
+                f"bonds at chi_max {int((engines['fp64'].bond_dims() == CHI).sum())}, |dn| {d:.2e}")
+    times = {"mixed": 0.0, "fp64": 0.0}
+    gpu_ms = {"mixed": 0.0, "fp64": 0.0}
+    flops = {}
+    for e in engines.values():
+        e.reset_stats()
+    for k in range(a.steps):
+        for name, e in engines.items():
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            e.step(1)
+            torch.cuda.synchronize()
+            times[name] += time.perf_counter() - t0
+        d = np.abs(engines["mixed"].rho()[:, 1, 1].real - engines["fp64"].rho()[:, 1, 1].real).max()
+        maxdiff = max(maxdiff, d)
+        log(f"timed step {k + 1}/{a.steps}: mixed {times['mixed']:.2f}s fp64 {times['fp64']:.2f}s |dn| {d:.2e}")
+    for name, e in engines.items():
+        gpu_ms[name] = e.gpu_ms()
+        flops[name] = e.stats()
+    su = a.steps * 2 * (N - 1)
+    rate = {k: su / v for k, v in times.items()}
+    bonds = engines["fp64"].bond_dims()
+    log("roofline micro-benchmark of the 2-site H_eff application")
+    ms32, fl = heff2_roofline(torch, 1)
+    ms64, _ = heff2_roofline(torch, 0)
+    ach32 = fl / (ms32 * 1e-3) / 1e12
+    ach64 = fl / (ms64 * 1e-3) / 1e12
+    cpu = None
+    if not a.no_cpu_baseline:
+        log(f"cpu baseline ({a.cpu_seconds:.0f} s budget)")
+        cpu = cpu_baseline(engines["fp64"].tensors(), a.cpu_seconds)
+    out = {
+        "metric": "2TDVP site-updates/s at chi=128, mixed vs FP64 @max|d<n>|<=1e-6; p50/p99 latency",
+        "value": rate["mixed"],
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * times["mixed"] / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 contractions + f64 Krylov/SVD",
+        "data": "synthetic (blinker initial state, rule Hamiltonian MPO)",
+        "config": {"workload": "2TDVP single instance: 41 cells, distance 2, interval [2,4], three centre "
+                               "cells alive, dt 0.1, chi_max 128, eps 1e-10",
+                   "ncells": N, "chi_max": CHI, "mpo_bond": int(Ws[N // 2].shape[2]),
+                   "bond_dims_after_warmup_max": int(bonds.max()),
+                   "bonds_at_chi_max": int((bonds == CHI).sum())},
+        "fp64_value": rate["fp64"],
+        "mixed_over_fp64": rate["mixed"] / rate["fp64"],
+        "max_abs_dn_mixed_vs_fp64": maxdiff,
+        "gpu_ms_per_step": {k: v / a.steps for k, v in gpu_ms.items()},
+        "matvecs_per_step": {k: v["matvecs"] / a.steps for k, v in flops.items()},
+        "contraction_tflops_per_s": {k: flops[k]["flops"] / times[k] / 1e12 for k in flops},
+        "roofline": {"bound": "mfma", "kernel": "2-site H_eff application (mixed, complex64 on FP32 MFMA)",
+                     "achieved": ach32, "peak": 157.3, "unit": "TFLOP/s", "frac": ach32 / 157.3,
+                     "traffic": None, "ms_per_launch": ms32, "flops_per_launch": fl,
+                     "fp64": {"achieved": ach64, "peak": 78.6, "frac": ach64 / 78.6, "ms_per_launch": ms64}},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -25,7 +25,7 @@ HALF_PI = np.pi / 2
 
 def merge_theta(Al: np.ndarray, Ar: np.ndarray) -> np.ndarray:
     """Theta[(p1 p2), a, c] = sum_b Al[p1,a,b] Ar[p2,b,c] (reference tdvp.py:278-280)."""
-    T = np.einsum("pab,qbc->pqac", Al, Ar)
+    T = np.einsum("pab,qbc->pqac", Al, Ar, optimize=True)
     return T.reshape(4, Al.shape[1], Ar.shape[2])
 
 
@@ -38,29 +38,29 @@ def merge_w(Wl: np.ndarray, Wr: np.ndarray) -> np.ndarray:
 
 def heff_apply(x: np.ndarray, L: np.ndarray, R: np.ndarray, W: np.ndarray) -> np.ndarray:
     """Matrix-free effective Hamiltonian (1-site or 2-site by the size of W)."""
-    t = np.einsum("awx,pac->pwxc", L, x)          # L first
-    t = np.einsum("pqwv,pwxc->qvxc", W, t)        # then W
-    return np.einsum("qvxc,cvy->qxy", t, R)       # then R
+    t = np.einsum("awx,pac->pwxc", L, x, optimize=True)          # L first
+    t = np.einsum("pqwv,pwxc->qvxc", W, t, optimize=True)        # then W
+    return np.einsum("qvxc,cvy->qxy", t, R, optimize=True)       # then R
 
 
 def bond_apply(C: np.ndarray, L: np.ndarray, R: np.ndarray) -> np.ndarray:
-    return np.einsum("awx,bwy,ab->xy", L, R, C)
+    return np.einsum("awx,bwy,ab->xy", L, R, C, optimize=True)
 
 
 def env_left(Lprev: np.ndarray, A: np.ndarray, W: np.ndarray) -> np.ndarray:
     """L_new[b,w',b*] = sum A[p,a,b] W[p,p',w,w'] conj(A[p',a*,b*]) L[a,w,a*]
     (reference tdvp.py:328-347, side='left')."""
-    t = np.einsum("awx,pab->pwxb", Lprev, A)
-    t = np.einsum("pqwv,pwxb->qvxb", W, t)
-    return np.einsum("qvxb,qxy->bvy", t, A.conj())
+    t = np.einsum("awx,pab->pwxb", Lprev, A, optimize=True)
+    t = np.einsum("pqwv,pwxb->qvxb", W, t, optimize=True)
+    return np.einsum("qvxb,qxy->bvy", t, A.conj(), optimize=True)
 
 
 def env_right(Rprev: np.ndarray, A: np.ndarray, W: np.ndarray) -> np.ndarray:
     """R_new[a,w,a*] = sum A[p,a,b] W[p,p',w,w'] conj(A[p',a*,b*]) R[b,w',b*]
     (reference tdvp.py:328-347, side='right')."""
-    t = np.einsum("bvy,pab->pavy", Rprev, A)
-    t = np.einsum("pqwv,pavy->qawy", W, t)
-    return np.einsum("qawy,qxy->awx", t, A.conj())
+    t = np.einsum("bvy,pab->pavy", Rprev, A, optimize=True)
+    t = np.einsum("pqwv,pavy->qawy", W, t, optimize=True)
+    return np.einsum("qawy,qxy->awx", t, A.conj(), optimize=True)
 
 
 def dense_heff(L: np.ndarray, R: np.ndarray, W: np.ndarray) -> np.ndarray:

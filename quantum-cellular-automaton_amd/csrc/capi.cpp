@@ -1,0 +1,508 @@
+// extern "C" boundary of libqca.so (declared in include/qca.h).
+#include "qca.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return QCA_OK;
+  } catch (const qca::Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return QCA_ERR_INTERNAL;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return QCA_ERR_INTERNAL;
+  }
+}
+
+// Growable per-device scratch for the primitive entry points.
+struct Scratch {
+  std::map<std::string, std::pair<void*, size_t>> bufs;
+  void* get(const std::string& name, size_t bytes) {
+    auto it = bufs.find(name);
+    if (it != bufs.end() && it->second.second >= bytes) return it->second.first;
+    if (it != bufs.end()) {
+      QCA_HIP(hipDeviceSynchronize());
+      QCA_HIP(hipFree(it->second.first));
+      bufs.erase(it);
+    }
+    void* p = nullptr;
+    QCA_HIP(hipMalloc(&p, bytes < 256 ? 256 : bytes));
+    bufs[name] = {p, bytes};
+    return p;
+  }
+  void release() {
+    for (auto& kv : bufs) (void)hipFree(kv.second.first);
+    bufs.clear();
+  }
+};
+std::map<int, Scratch> g_scratch;
+
+Scratch& scratch() {
+  int dev = 0;
+  QCA_HIP(hipGetDevice(&dev));
+  return g_scratch[dev];
+}
+
+template <typename T>
+T* sbuf(const char* name, long n) {
+  return reinterpret_cast<T*>(scratch().get(name, sizeof(T) * (size_t)std::max<long>(n, 1)));
+}
+
+template <typename T>
+qca::ContractWs<T> contract_ws(long tsize, long outsize) {
+  qca::ContractWs<T> w;
+  w.T1 = sbuf<qca::cplx<T>>("T1", tsize);
+  w.U = sbuf<qca::cplx<T>>("U", tsize);
+  w.splitk_elems = 16 * std::max(outsize, tsize / 4);
+  w.splitk = sbuf<qca::cplx<T>>("splitk", w.splitk_elems);
+  return w;
+}
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+qca::LanczosWork lanczos_ws(long n, qca::LanczosCtrl** ctrl_host, int max_m = qca::LZ_MAXK) {
+  qca::LanczosWork lw;
+  lw.max_n = n;
+  lw.max_m = max_m;
+  lw.max_blocks = qca::lanczos_nblk(n);
+  lw.ctrl = sbuf<qca::LanczosCtrl>("lz_ctrl", 1);
+  lw.V = sbuf<qca::c128>("lz_V", (long)max_m * n);
+  lw.w64 = sbuf<qca::c128>("lz_w", n);
+  lw.part = sbuf<qca::c128>("lz_part", (long)lw.max_blocks * qca::LZ_MAXK);
+  lw.part_norm = sbuf<double>("lz_pnorm", lw.max_blocks);
+  lw.coef1 = sbuf<qca::c128>("lz_c1", qca::LZ_MAXK);
+  lw.coef2 = sbuf<qca::c128>("lz_c2", qca::LZ_MAXK);
+  static thread_local qca::LanczosCtrl* host = nullptr;
+  if (!host) QCA_HIP(hipHostMalloc((void**)&host, sizeof(qca::LanczosCtrl)));
+  *ctrl_host = host;
+  return lw;
+}
+
+// Runs the Krylov loop: mv(j, V_j) must enqueue w = H V_j and the iteration tail.
+template <typename MV>
+int krylov_loop(qca::c128* x, long n, qca::LanczosWork& lw, qca::LanczosCtrl* host, MV&& mv,
+                hipStream_t s) {
+  qca::lanczos_begin(x, n, lw, s);
+  int j = 0, chunk = 8;
+  while (true) {
+    for (int c = 0; c < chunk && j < lw.max_m - 2; ++c, ++j) mv(j, lw.V + (long)j * n);
+    QCA_HIP(hipMemcpyAsync(host, lw.ctrl, sizeof(qca::LanczosCtrl), hipMemcpyDeviceToHost, s));
+    QCA_HIP(hipStreamSynchronize(s));
+    if (host->done) break;
+    QCA_CHECK(j < lw.max_m - 2, "Krylov exponential did not converge");
+    chunk = 4;
+  }
+  qca::lanczos_finish(n, x, lw, s);
+  QCA_HIP(hipStreamSynchronize(s));
+  return host->m;
+}
+
+}  // namespace
+
+struct qca_engine {
+  qca::Engine* eng = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double gpu_ms = 0;
+};
+
+extern "C" {
+
+int qca_version(void) { return 10000; }
+
+const char* qca_last_error(void) { return g_last_error.c_str(); }
+
+int qca_device_count(int* count) {
+  return guarded([&] {
+    QCA_CHECK(count != nullptr, "null count");
+    QCA_HIP(hipGetDeviceCount(count));
+  });
+}
+
+int qca_engine_create(const qca_engine_params* params, const double* const* W, const int* Wdl,
+                      const int* Wdr, const double* const* A, const int* chis, qca_engine** out) {
+  return guarded([&] {
+    QCA_CHECK(params && W && Wdl && Wdr && A && chis && out, "null argument");
+    const int n = params->ncells;
+    QCA_CHECK(n >= 2 && n <= 4096, "ncells out of range");
+    qca::EngineParams p;
+    p.ncells = n;
+    p.algorithm = params->algorithm;
+    p.precision = params->precision;
+    p.chi_max = params->chi_max;
+    p.dt = params->dt;
+    p.eps = params->eps;
+    p.krylov_tol = params->krylov_tol > 0 ? params->krylov_tol : 1e-14;
+    std::vector<const qca::c128*> Wv(n), Av(n);
+    for (int i = 0; i < n; ++i) {
+      QCA_CHECK(W[i] && A[i], "null tensor pointer");
+      Wv[i] = reinterpret_cast<const qca::c128*>(W[i]);
+      Av[i] = reinterpret_cast<const qca::c128*>(A[i]);
+    }
+    std::vector<int> dl(Wdl, Wdl + n), dr(Wdr, Wdr + n), ch(chis, chis + n + 1);
+    auto* h = new qca_engine();
+    try {
+      h->eng = new qca::Engine(params->device, p, Wv, dl, dr, Av, ch);
+      QCA_HIP(hipEventCreate(&h->ev0));
+      QCA_HIP(hipEventCreate(&h->ev1));
+    } catch (...) {
+      delete h->eng;
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int qca_engine_destroy(qca_engine* e) {
+  return guarded([&] {
+    if (!e) return;
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    delete e->eng;
+    delete e;
+  });
+}
+
+int qca_engine_step(qca_engine* e, int nsteps) {
+  return guarded([&] {
+    QCA_CHECK(e && e->eng, "null engine");
+    QCA_CHECK(nsteps >= 0, "negative step count");
+    for (int i = 0; i < nsteps; ++i) {
+      QCA_HIP(hipEventRecord(e->ev0, e->eng->stream()));
+      e->eng->step();
+      QCA_HIP(hipEventRecord(e->ev1, e->eng->stream()));
+      QCA_HIP(hipEventSynchronize(e->ev1));
+      float ms = 0;
+      QCA_HIP(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+      e->gpu_ms += ms;
+    }
+  });
+}
+
+int qca_engine_measure(qca_engine* e, double* rho_out) {
+  return guarded([&] {
+    QCA_CHECK(e && e->eng && rho_out, "null argument");
+    e->eng->measure(reinterpret_cast<qca::c128*>(rho_out));
+  });
+}
+
+int qca_engine_bond_dims(qca_engine* e, int* out) {
+  return guarded([&] {
+    QCA_CHECK(e && e->eng && out, "null argument");
+    for (int i = 0; i <= e->eng->ncells(); ++i) out[i] = e->eng->chi(i);
+  });
+}
+
+int qca_engine_get_site(qca_engine* e, int site, double* out) {
+  return guarded([&] {
+    QCA_CHECK(e && e->eng && out, "null argument");
+    e->eng->get_site(site, reinterpret_cast<qca::c128*>(out));
+  });
+}
+
+int qca_engine_stats(qca_engine* e, qca_stats* out) {
+  return guarded([&] {
+    QCA_CHECK(e && e->eng && out, "null argument");
+    const qca::StepStats& s = e->eng->stats();
+    out->matvecs = s.matvecs;
+    out->matvec_2site = s.matvec_2site;
+    out->matvec_1site = s.matvec_1site;
+    out->matvec_bond = s.matvec_bond;
+    out->site_updates = s.site_updates;
+    out->svd_sweeps = s.svd_sweeps;
+    out->lanczos_limit_hits = s.lanczos_limit_hits;
+    out->flops = s.flops;
+  });
+}
+
+int qca_engine_reset_stats(qca_engine* e) {
+  return guarded([&] {
+    QCA_CHECK(e && e->eng, "null engine");
+    e->eng->reset_stats();
+    e->gpu_ms = 0;
+  });
+}
+
+int qca_engine_gpu_ms(qca_engine* e, double* ms) {
+  return guarded([&] {
+    QCA_CHECK(e && ms, "null argument");
+    *ms = e->gpu_ms;
+  });
+}
+
+// ------------------------------------------------------------------ primitives
+
+int qca_heff_apply(int precision, int kind, const void* x, void* y, const void* L, const void* R,
+                   const double* W_host, int Din, int Dout, int chil, int chir, void* stream) {
+  return guarded([&] {
+    QCA_CHECK(precision == 0 || precision == 1, "precision must be 0 or 1");
+    QCA_CHECK(kind >= 0 && kind <= 2, "kind must be 0, 1 or 2");
+    QCA_CHECK(x && y && L && R, "null tensor");
+    QCA_CHECK(chil >= 1 && chir >= 1 && chil <= 512 && chir <= 512, "bond out of range");
+    QCA_CHECK(Din >= 1 && Dout >= 1 && Din <= 16 && Dout <= 16, "MPO bond out of range");
+    QCA_CHECK(kind == 2 ? Din == Dout : W_host != nullptr, "bad MPO arguments");
+    hipStream_t s = as_stream(stream);
+    const int dq = kind == 0 ? 4 : (kind == 1 ? 2 : 1);
+    const long tsize = (long)std::max(dq, 1) * std::max(Din, Dout) * chil * chir;
+    qca::WSparsePair wp;
+    if (kind != 2) wp.upload(qca::build_sparse(reinterpret_cast<const qca::c128*>(W_host), dq, Din, Dout, false), dq * Din);
+    try {
+      if (precision == 0) {
+        auto ws = contract_ws<double>(tsize, (long)dq * chil * chir);
+        qca::heff_apply<double>(kind, (const qca::c128*)x, (qca::c128*)y, (const qca::c128*)L,
+                                (const qca::c128*)R, kind == 2 ? nullptr : &wp.d64, Din, Dout, chil,
+                                chir, ws, nullptr, s);
+      } else {
+        auto ws = contract_ws<float>(tsize, (long)dq * chil * chir);
+        qca::heff_apply<float>(kind, (const qca::c64*)x, (qca::c64*)y, (const qca::c64*)L,
+                               (const qca::c64*)R, kind == 2 ? nullptr : &wp.d32, Din, Dout, chil,
+                               chir, ws, nullptr, s);
+      }
+      QCA_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+      wp.release();
+      throw;
+    }
+    wp.release();
+  });
+}
+
+int qca_env_update(int precision, int side, const void* A, const void* Eprev, void* Enew,
+                   const double* W_host, int Dl, int Dr, int chil, int chir, void* stream) {
+  return guarded([&] {
+    QCA_CHECK(precision == 0 || precision == 1, "precision must be 0 or 1");
+    QCA_CHECK(side == 0 || side == 1, "side must be 0 (left) or 1 (right)");
+    QCA_CHECK(A && Eprev && Enew && W_host, "null tensor");
+    QCA_CHECK(chil >= 1 && chir >= 1 && chil <= 512 && chir <= 512, "bond out of range");
+    QCA_CHECK(Dl >= 1 && Dr >= 1 && Dl <= 16 && Dr <= 16, "MPO bond out of range");
+    hipStream_t s = as_stream(stream);
+    const bool left = side == 0;
+    qca::WSparsePair wp;
+    wp.upload(qca::build_sparse(reinterpret_cast<const qca::c128*>(W_host), 2, Dl, Dr, !left),
+              left ? 2 * Dl : 2 * Dr);
+    const int Din = left ? Dl : Dr, Dout = left ? Dr : Dl;
+    const long tsize = 2L * std::max(Dl, Dr) * chil * chir;
+    const long osize = (long)std::max(chil, chir) * std::max(Dl, Dr) * std::max(chil, chir);
+    try {
+      if (precision == 0) {
+        auto ws = contract_ws<double>(tsize, osize);
+        qca::env_update<double>(left, (const qca::c128*)A, (const qca::c128*)Eprev,
+                                (qca::c128*)Enew, &wp.d64, Din, Dout, chil, chir, ws, s);
+      } else {
+        auto ws = contract_ws<float>(tsize, osize);
+        qca::env_update<float>(left, (const qca::c64*)A, (const qca::c64*)Eprev, (qca::c64*)Enew,
+                               &wp.d32, Din, Dout, chil, chir, ws, s);
+      }
+      QCA_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+      wp.release();
+      throw;
+    }
+    wp.release();
+  });
+}
+
+int qca_krylov_expm(int precision, int kind, void* x, const void* L, const void* R,
+                    const double* W_host, int Din, int Dout, int chil, int chir, double delta,
+                    double tol, int* iters, void* stream) {
+  return guarded([&] {
+    QCA_CHECK(precision == 0 || precision == 1, "precision must be 0 or 1");
+    QCA_CHECK(kind >= 0 && kind <= 2, "kind must be 0, 1 or 2");
+    QCA_CHECK(x && L && R, "null tensor");
+    QCA_CHECK(chil >= 1 && chir >= 1 && chil <= 512 && chir <= 512, "bond out of range");
+    QCA_CHECK(Din >= 1 && Dout >= 1 && Din <= 16 && Dout <= 16, "MPO bond out of range");
+    QCA_CHECK(kind == 2 ? Din == Dout : W_host != nullptr, "bad MPO arguments");
+    hipStream_t s = as_stream(stream);
+    const int dq = kind == 0 ? 4 : (kind == 1 ? 2 : 1);
+    const long n = (long)dq * chil * chir;
+    const long tsize = (long)dq * std::max(Din, Dout) * chil * chir;
+    qca::WSparsePair wp;
+    if (kind != 2) wp.upload(qca::build_sparse(reinterpret_cast<const qca::c128*>(W_host), dq, Din, Dout, false), dq * Din);
+    qca::LanczosCtrl* host = nullptr;
+    qca::LanczosWork lw = lanczos_ws(n, &host);
+    const double tau = 1.5707963267948966 * delta;
+    const double tl = tol > 0 ? tol : 1e-14;
+    try {
+      int m = 0;
+      if (precision == 0) {
+        auto ws = contract_ws<double>(tsize, n);
+        qca::c128* y = sbuf<qca::c128>("kr_y", n);
+        m = krylov_loop((qca::c128*)x, n, lw, host, [&](int j, const qca::c128* vj) {
+          qca::heff_apply<double>(kind, vj, y, (const qca::c128*)L, (const qca::c128*)R,
+                                  kind == 2 ? nullptr : &wp.d64, Din, Dout, chil, chir, ws,
+                                  qca::lanczos_done_flag(lw), s);
+          qca::lanczos_iteration_tail<double>(y, n, j, tau, tl, lw, s);
+        }, s);
+      } else {
+        auto ws = contract_ws<float>(tsize, n);
+        qca::c64* x32 = sbuf<qca::c64>("kr_x32", n);
+        qca::c64* y = sbuf<qca::c64>("kr_y32", n);
+        m = krylov_loop((qca::c128*)x, n, lw, host, [&](int j, const qca::c128* vj) {
+          qca::launch_convert_c128_c64(vj, x32, n, s, qca::lanczos_done_flag(lw));
+          qca::heff_apply<float>(kind, x32, y, (const qca::c64*)L, (const qca::c64*)R,
+                                 kind == 2 ? nullptr : &wp.d32, Din, Dout, chil, chir, ws,
+                                 qca::lanczos_done_flag(lw), s);
+          qca::lanczos_iteration_tail<float>(y, n, j, tau, tl, lw, s);
+        }, s);
+      }
+      if (iters) *iters = m;
+    } catch (...) {
+      wp.release();
+      throw;
+    }
+    wp.release();
+  });
+}
+
+int qca_svd_split(const void* theta, void* Al, void* Ar, int chil, int chir, int chi_max,
+                  double eps, int right_sweep, int* k, double* sigma_host, void* stream) {
+  return guarded([&] {
+    QCA_CHECK(theta && Al && Ar && k, "null argument");
+    QCA_CHECK(chil >= 1 && chir >= 1 && chil <= 256 && chir <= 256, "bond out of range");
+    QCA_CHECK(chi_max >= 1, "chi_max must be positive");
+    hipStream_t s = as_stream(stream);
+    const int rows = right_sweep ? 2 * chil : 2 * chir;
+    const int cols = right_sweep ? 2 * chir : 2 * chil;
+    const int rank_cap = std::min(2 * chil, 2 * chir);
+    const qca::c128* th = (const qca::c128*)theta;
+    qca::c128* X = sbuf<qca::c128>("svd_X", 4L * chil * chir);
+    double* sigma = sbuf<double>("svd_sigma", cols);
+    int* perm = sbuf<int>("svd_perm", cols);
+    qca::SvdResult* res = sbuf<qca::SvdResult>("svd_res", 1);
+    qca::launch_theta_to_svd(th, X, chil, chir, right_sweep ? 0 : 1, s);
+    qca::launch_jacobi_svd(X, rows, cols, sigma, perm, res, chi_max, eps, rank_cap, s);
+    qca::SvdResult hres;
+    QCA_HIP(hipMemcpyAsync(&hres, res, sizeof(hres), hipMemcpyDeviceToHost, s));
+    QCA_HIP(hipStreamSynchronize(s));
+    const int kk = hres.k;
+    QCA_CHECK(kk >= 1, "empty truncation");
+    qca::ContractWs<double> ws = contract_ws<double>(4L * chil * chir, 4L * chil * chir);
+    const double inv = 1.0 / hres.norm_kept;
+    if (right_sweep) {
+      qca::launch_gather_isometry(X, perm, sigma, (qca::c128*)Al, rows, chil, kk, 0, s);
+      qca::GemmDesc g = qca::gemm_desc(kk, chir, 2 * chil);
+      g.K2 = chil;
+      g.NB2 = 2;
+      g.a_m = 1; g.a_k1 = (long)chil * kk; g.a_k2 = kk; g.conj_a = 1;
+      g.b_k1 = 2L * chil * chir; g.b_k2 = chir; g.b_n = 1; g.b_b2 = (long)chil * chir;
+      g.c_m = chir; g.c_n = 1; g.c_b2 = (long)kk * chir;
+      g.alpha_re = inv;
+      qca::gemm_t<double>(g, (const qca::c128*)Al, th, (qca::c128*)Ar, s, ws);
+    } else {
+      qca::launch_gather_isometry(X, perm, sigma, (qca::c128*)Ar, rows, chir, kk, 1, s);
+      qca::GemmDesc g = qca::gemm_desc(chil, kk, 2 * chir);
+      g.K2 = chir;
+      g.NB2 = 2;
+      g.a_m = chir; g.a_k1 = (long)chil * chir; g.a_k2 = 1; g.a_b2 = 2L * chil * chir;
+      g.b_k1 = (long)kk * chir; g.b_k2 = 1; g.b_n = chir; g.conj_b = 1;
+      g.c_m = kk; g.c_n = 1; g.c_b2 = (long)chil * kk;
+      g.alpha_re = inv;
+      qca::gemm_t<double>(g, th, (const qca::c128*)Ar, (qca::c128*)Al, s, ws);
+    }
+    if (sigma_host)
+      QCA_HIP(hipMemcpyAsync(sigma_host, sigma, sizeof(double) * rank_cap, hipMemcpyDeviceToHost, s));
+    QCA_HIP(hipStreamSynchronize(s));
+    *k = kk;
+  });
+}
+
+int qca_qr_site(int side, const void* A, int chil, int chir, int target, void* Anew, void* C,
+                void* stream) {
+  return guarded([&] {
+    QCA_CHECK(side == 0 || side == 1, "side must be 0 or 1");
+    QCA_CHECK(A && Anew && C, "null tensor");
+    QCA_CHECK(chil >= 1 && chir >= 1 && chil <= 512 && chir <= 512 && target >= 1, "bad shape");
+    hipStream_t s = as_stream(stream);
+    const int rows = side == 0 ? 2 * chil : 2 * chir;
+    const int cols = side == 0 ? chir : chil;
+    qca::c128* X = sbuf<qca::c128>("qr_X", (long)rows * cols);
+    qca::c128* Q = sbuf<qca::c128>("qr_Q", (long)rows * std::min(target, rows));
+    qca::c128* tau = sbuf<qca::c128>("qr_tau", std::min(rows, cols));
+    const int qcols = std::min(target, rows);
+    if (side == 0) qca::launch_to_colmajor((const qca::c128*)A, X, rows, cols, s);
+    else qca::launch_site_to_rightqr((const qca::c128*)A, X, chil, chir, s);
+    qca::launch_householder_qr(X, rows, cols, tau, Q, qcols, s);
+    if (side == 0) qca::launch_qr_gather_left(Q, X, (qca::c128*)Anew, (qca::c128*)C, rows, cols, qcols, target, s);
+    else qca::launch_qr_gather_right(Q, X, (qca::c128*)Anew, (qca::c128*)C, chir, cols, qcols, target, s);
+    QCA_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int qca_exact_apply_h(int precision, const void* psi, void* out, int ncells, int distance, int lo,
+                      int hi, void* stream) {
+  return guarded([&] {
+    QCA_CHECK(precision == 0 || precision == 1, "precision must be 0 or 1");
+    QCA_CHECK(psi && out, "null vector");
+    QCA_CHECK(ncells >= 1 && ncells <= 34, "ncells out of range for the state vector");
+    QCA_CHECK(distance >= 1, "distance must be positive");
+    hipStream_t s = as_stream(stream);
+    if (precision == 0)
+      qca::launch_exact_apply<double>((const qca::c128*)psi, (qca::c128*)out, ncells, distance, lo, hi, s);
+    else
+      qca::launch_exact_apply<float>((const qca::c64*)psi, (qca::c64*)out, ncells, distance, lo, hi, s);
+    QCA_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int qca_exact_expm(void* psi, int ncells, int distance, int lo, int hi, double dt, double tol,
+                   int* iters, void* stream) {
+  return guarded([&] {
+    QCA_CHECK(psi, "null vector");
+    QCA_CHECK(ncells >= 2 && ncells <= 28, "ncells out of range for the Krylov exponential (2..28)");
+    QCA_CHECK(distance >= 1, "distance must be positive");
+    hipStream_t s = as_stream(stream);
+    const long n = 1L << ncells;
+    // Krylov basis capped at 16 GiB (at least 24 vectors)
+    const int max_m = (int)std::max<long>(24, std::min<long>(qca::LZ_MAXK, (16L << 30) / (16L * n)));
+    qca::LanczosCtrl* host = nullptr;
+    qca::LanczosWork lw = lanczos_ws(n, &host, max_m);
+    qca::c128* y = sbuf<qca::c128>("ex_y", n);
+    const double tau = 1.5707963267948966 * dt;
+    const double tl = tol > 0 ? tol : 1e-14;
+    const int m = krylov_loop((qca::c128*)psi, n, lw, host, [&](int j, const qca::c128* vj) {
+      qca::launch_exact_apply<double>(vj, y, ncells, distance, lo, hi, s);
+      qca::lanczos_iteration_tail<double>(y, n, j, tau, tl, lw, s);
+    }, s);
+    if (iters) *iters = m;
+  });
+}
+
+int qca_exact_rho(const void* psi, int ncells, double* rho_host, void* stream) {
+  return guarded([&] {
+    QCA_CHECK(psi && rho_host, "null argument");
+    QCA_CHECK(ncells >= 1 && ncells <= 34, "ncells out of range for the state vector");
+    hipStream_t s = as_stream(stream);
+    double* part = sbuf<double>("rho_part", 4L * ncells * qca::exact_rho_blocks(ncells));
+    qca::c128* rho = sbuf<qca::c128>("rho_out", 4L * ncells);
+    qca::launch_exact_rho((const qca::c128*)psi, ncells, part, rho, s);
+    QCA_HIP(hipMemcpyAsync(rho_host, rho, sizeof(qca::c128) * 4L * ncells, hipMemcpyDeviceToHost, s));
+    QCA_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int qca_release_workspace(void) {
+  return guarded([&] {
+    QCA_HIP(hipDeviceSynchronize());
+    scratch().release();
+  });
+}
+
+}  // extern "C"

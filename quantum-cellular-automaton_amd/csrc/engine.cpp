@@ -1,0 +1,565 @@
+// Device-resident TDVP engine (host orchestration of the gfx950 kernels).
+//
+// Follows reference algorithms/tdvp.py in gauge-consistent form:
+//   * 2TDVP (tdvp.py:107-145): per bond merge -> Krylov exp(+dt/2) -> SVD split
+//     and truncation -> left/right environment update -> 1-site exp(-dt/2).
+//   * 1TDVP (tdvp.py:65-105): 1-site exp(+dt/2) -> complete QR truncated/padded
+//     to the target bond -> environment update -> bond exp(-dt/2) -> absorb.
+// The reference's per-step make_site_canonical(0) (tdvp.py:54) only changes the
+// gauge of tensors that are already right-canonical and leaves the cached right
+// environments stale; here the state stays in the gauge the environments were
+// built in, so no re-canonicalisation is done between steps, and measurement
+// does not move the orthogonality centre.
+//
+// Every tensor lives in HBM for the lifetime of the engine.  The host only
+// reads back the truncation rank after each SVD and the convergence flag of
+// each Krylov exponential.
+#include "engine.hpp"
+
+#include <type_traits>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace qca {
+
+namespace {
+constexpr double kHalfPi = 1.5707963267948966;
+
+int sat_pow2_min(int e_left, int e_right, int cap) {
+  // min(2^e_left, 2^e_right, cap) without overflow (reference tdvp.py:31-33)
+  long v = cap;
+  if (e_left < 31) v = std::min<long>(v, 1L << e_left);
+  if (e_right < 31) v = std::min<long>(v, 1L << e_right);
+  return (int)v;
+}
+
+}  // namespace
+
+template <typename T>
+T* Engine::dmalloc(long n) {
+  void* p = nullptr;
+  if (n <= 0) n = 1;
+  QCA_HIP(hipMalloc(&p, sizeof(T) * (size_t)n));
+  allocs_.push_back(p);
+  return reinterpret_cast<T*>(p);
+}
+
+Engine::Engine(int device, const EngineParams& p, const std::vector<const c128*>& W_host,
+               const std::vector<int>& Wdl, const std::vector<int>& Wdr,
+               const std::vector<const c128*>& A_host, const std::vector<int>& chis)
+    : p_(p), N_(p.ncells), device_(device) {
+  QCA_CHECK(N_ >= 2, "at least 2 cells are required");
+  QCA_CHECK((int)W_host.size() == N_ && (int)Wdl.size() == N_ && (int)Wdr.size() == N_,
+            "one MPO tensor per cell");
+  QCA_CHECK((int)A_host.size() == N_ && (int)chis.size() == N_ + 1, "one MPS tensor per cell");
+  QCA_CHECK(chis[0] == 1 && chis[N_] == 1, "open boundary bonds must be 1");
+  QCA_CHECK(p.algorithm == 1 || p.algorithm == 2, "algorithm must be 1 (1TDVP) or 2 (2TDVP)");
+  QCA_CHECK(p.precision == PREC_FP64 || p.precision == PREC_MIXED, "unknown precision");
+  QCA_CHECK(p.chi_max >= 1 && p.chi_max <= 256, "max bond dimension must be in 1..256");
+  QCA_CHECK(Wdl[0] == 1 && Wdr[N_ - 1] == 1, "MPO boundary bonds must be 1");
+  for (int i = 0; i + 1 < N_; ++i) QCA_CHECK(Wdr[i] == Wdl[i + 1], "MPO bond mismatch");
+  for (int i = 0; i < N_; ++i) QCA_CHECK(chis[i] >= 1 && chis[i] <= 256, "MPS bond out of range");
+  QCA_HIP(hipSetDevice(device));
+  QCA_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+  own_stream_ = true;
+
+  chi_ = chis;
+  cap_.resize(N_ + 1);
+  target_.resize(N_ + 1);
+  for (int i = 0; i <= N_; ++i) {
+    target_[i] = sat_pow2_min(i, N_ - i, p.chi_max);
+    cap_[i] = std::max(target_[i], chi_[i]);
+  }
+  Dl_ = Wdl;
+  Dr_ = Wdr;
+  Dmax_ = 1;
+  for (int i = 0; i < N_; ++i) Dmax_ = std::max({Dmax_, Dl_[i], Dr_[i]});
+  QCA_CHECK(Dmax_ <= 16, "MPO bond dimension above 16");
+
+  // ---- MPO in sparse form, both precisions
+  Wf_.resize(N_);
+  Wb_.resize(N_);
+  for (int i = 0; i < N_; ++i) {
+    Wf_[i].upload(build_sparse(W_host[i], 2, Dl_[i], Dr_[i], false), 2 * Dl_[i]);
+    Wb_[i].upload(build_sparse(W_host[i], 2, Dl_[i], Dr_[i], true), 2 * Dr_[i]);
+  }
+  if (p.algorithm == 2) {
+    W2f_.resize(N_ - 1);
+    for (int s = 0; s + 1 < N_; ++s) {
+      auto w2 = merge_mpo(W_host[s], W_host[s + 1], Dl_[s], Dr_[s], Dr_[s + 1]);
+      W2f_[s].upload(build_sparse(w2.data(), 4, Dl_[s], Dr_[s + 1], false), 4 * Dl_[s]);
+    }
+  }
+
+  // ---- state
+  const bool mixed = p.precision == PREC_MIXED;
+  const size_t esz = mixed ? sizeof(c64) : sizeof(c128);
+  A_.resize(N_);
+  for (int i = 0; i < N_; ++i) {
+    A_[i] = dmalloc<c128>(2L * cap_[i] * cap_[i + 1]);
+    QCA_HIP(hipMemcpy(A_[i], A_host[i], sizeof(c128) * 2L * chi_[i] * chi_[i + 1],
+                      hipMemcpyHostToDevice));
+  }
+  L_.assign(N_, nullptr);
+  R_.assign(N_, nullptr);
+  for (int i = 0; i < N_; ++i) {
+    L_[i] = dmalloc<unsigned char>((long)esz * cap_[i + 1] * Dr_[i] * cap_[i + 1]);
+    R_[i] = dmalloc<unsigned char>((long)esz * cap_[i] * Dl_[i] * cap_[i]);
+  }
+  one64_ = dmalloc<c128>(1);
+  one32_ = dmalloc<c64>(1);
+  {
+    c128 o{1.0, 0.0};
+    c64 o32{1.0f, 0.0f};
+    QCA_HIP(hipMemcpy(one64_, &o, sizeof(o), hipMemcpyHostToDevice));
+    QCA_HIP(hipMemcpy(one32_, &o32, sizeof(o32), hipMemcpyHostToDevice));
+  }
+
+  // ---- workspaces
+  long cmax = 1;
+  for (int i = 0; i <= N_; ++i) cmax = std::max<long>(cmax, cap_[i]);
+  long nmax = 1;  // longest Krylov vector
+  for (int s = 0; s < N_; ++s) nmax = std::max<long>(nmax, 2L * cap_[s] * cap_[s + 1]);
+  for (int s = 0; s + 2 <= N_; ++s) nmax = std::max<long>(nmax, 4L * cap_[s] * cap_[s + 2]);
+  for (int b = 0; b <= N_; ++b) nmax = std::max<long>(nmax, (long)cap_[b] * cap_[b]);
+  const long tmax = 4L * Dmax_ * cmax * cmax;
+  theta_ = dmalloc<c128>(4 * cmax * cmax);
+  theta2_ = dmalloc<c128>(4 * cmax * cmax);
+  svdX_ = dmalloc<c128>(4 * cmax * cmax);
+  qrQ_ = dmalloc<c128>(4 * cmax * cmax);
+  qrTau_ = dmalloc<c128>(2 * cmax);
+  cbond_ = dmalloc<c128>(cmax * cmax * 2);
+  T_ = dmalloc<c128>(tmax);
+  U_ = dmalloc<c128>(tmax);
+  ws_elems_ = std::max<long>(16 * nmax, 4 * Dmax_ * cmax * cmax);
+  ws_ = dmalloc<c128>(ws_elems_);
+  ws64_.T1 = reinterpret_cast<c128*>(T_);
+  ws64_.U = reinterpret_cast<c128*>(U_);
+  ws64_.splitk = reinterpret_cast<c128*>(ws_);
+  ws64_.splitk_elems = ws_elems_;
+  ws32_.T1 = reinterpret_cast<c64*>(T_);
+  ws32_.U = reinterpret_cast<c64*>(U_);
+  ws32_.splitk = reinterpret_cast<c64*>(ws_);
+  ws32_.splitk_elems = 2 * ws_elems_;
+  if (mixed) {
+    x32_ = dmalloc<c64>(std::max(nmax, 2 * cmax * cmax));
+    y32_ = dmalloc<c64>(nmax);
+    a32_ = dmalloc<c64>(2 * cmax * cmax);
+  }
+  y64_ = dmalloc<c128>(nmax);
+  sigma_ = dmalloc<double>(2 * cmax);
+  perm_ = dmalloc<int>(2 * cmax);
+  svdres_ = dmalloc<SvdResult>(1);
+  QCA_HIP(hipHostMalloc((void**)&svdres_host_, sizeof(SvdResult)));
+  QCA_HIP(hipHostMalloc((void**)&ctrl_host_, sizeof(LanczosCtrl)));
+  lw_.max_n = nmax;
+  lw_.max_blocks = lanczos_nblk(nmax);
+  lw_.ctrl = dmalloc<LanczosCtrl>(1);
+  lw_.V = dmalloc<c128>((long)LZ_MAXK * nmax);
+  lw_.w64 = dmalloc<c128>(nmax);
+  lw_.part = dmalloc<c128>((long)lw_.max_blocks * LZ_MAXK);
+  lw_.part_norm = dmalloc<double>(lw_.max_blocks);
+  lw_.coef1 = dmalloc<c128>(LZ_MAXK);
+  lw_.coef2 = dmalloc<c128>(LZ_MAXK);
+  measE_ = dmalloc<c128>(cmax * cmax);
+  measT_ = dmalloc<c128>(2 * cmax * cmax);
+  rho_ = dmalloc<c128>(4L * N_);
+
+  // ---- TDVP construction (reference tdvp.py:20-39): canonicalise, then the
+  // right environments of every site.
+  canonicalize_initial();
+  for (int site = N_ - 1; site >= 1; --site) update_env(false, site);
+  sync();
+}
+
+Engine::~Engine() {
+  if (s_) (void)hipStreamSynchronize(s_);
+  for (auto& w : Wf_) w.release();
+  for (auto& w : Wb_) w.release();
+  for (auto& w : W2f_) w.release();
+  for (void* p : allocs_) (void)hipFree(p);
+  if (svdres_host_) (void)hipHostFree(svdres_host_);
+  if (ctrl_host_) (void)hipHostFree(ctrl_host_);
+  if (own_stream_ && s_) (void)hipStreamDestroy(s_);
+}
+
+void Engine::sync() { QCA_HIP(hipStreamSynchronize(s_)); }
+
+template <>
+void Engine::gemm<double>(const GemmDesc& d, const c128* A, const c128* B, c128* C) {
+  gemm_t<double>(d, A, B, C, s_, ws64_);
+}
+template <>
+void Engine::gemm<float>(const GemmDesc& d, const c64* A, const c64* B, c64* C) {
+  gemm_t<float>(d, A, B, C, s_, ws32_);
+}
+
+template <>
+cplx<double>* Engine::Lenv<cplx<double>>(int site) const {
+  return site < 0 ? one64_ : reinterpret_cast<c128*>(L_[site]);
+}
+template <>
+cplx<float>* Engine::Lenv<cplx<float>>(int site) const {
+  return site < 0 ? one32_ : reinterpret_cast<c64*>(L_[site]);
+}
+template <>
+cplx<double>* Engine::Renv<cplx<double>>(int site) const {
+  return site >= N_ ? one64_ : reinterpret_cast<c128*>(R_[site]);
+}
+template <>
+cplx<float>* Engine::Renv<cplx<float>>(int site) const {
+  return site >= N_ ? one32_ : reinterpret_cast<c64*>(R_[site]);
+}
+
+// ---------------------------------------------------------------- H_eff
+// kind 0: 2-site at bond (site, site+1), x = Theta[q][a][c] (4, chil, chir)
+// kind 1: 1-site at site, x = A[p][a][b] (2, chil, chir)
+// kind 2: bond between site-1 and site, x = C[a][b] (chil, chir)
+template <typename T>
+void Engine::heff(int kind, int site, const cplx<T>* x, cplx<T>* y, int chil, int chir) {
+  const int* skip = lanczos_done_flag(lw_);
+  const bool dbl = std::is_same<T, double>::value;
+  if (kind == 2) {
+    const int D = Dl_[site];
+    qca::heff_apply<T>(2, x, y, Lenv<cplx<T>>(site - 1), Renv<cplx<T>>(site), nullptr, D, D, chil,
+                       chir, ws_for<T>(), skip, s_);
+    return;
+  }
+  const WSparsePair& wp = kind == 0 ? W2f_[site] : Wf_[site];
+  const WSparseDev<T>* W;
+  if constexpr (std::is_same<T, double>::value) W = &wp.d64; else W = &wp.d32;
+  (void)dbl;
+  const int Din = Dl_[site];
+  const int Dout = kind == 0 ? Dr_[site + 1] : Dr_[site];
+  qca::heff_apply<T>(kind, x, y, Lenv<cplx<T>>(site - 1),
+                     Renv<cplx<T>>(kind == 0 ? site + 2 : site + 1), W, Din, Dout, chil, chir,
+                     ws_for<T>(), skip, s_);
+}
+
+template <typename T>
+void Engine::lanczos_run(int kind, int site, c128* x, long n, int chil, int chir, double delta) {
+  const double tau = kHalfPi * delta;
+  lanczos_begin(x, n, lw_, s_);
+  int j = 0;
+  int chunk = std::max(2, kpred_[kind]);
+  while (true) {
+    for (int c = 0; c < chunk && j < lw_.max_m - 2; ++c, ++j) {
+      const c128* vj = lw_.V + (long)j * n;
+      if constexpr (std::is_same<T, double>::value) {
+        heff<double>(kind, site, vj, y64_, chil, chir);
+        lanczos_iteration_tail<double>(y64_, n, j, tau, p_.krylov_tol, lw_, s_);
+      } else {
+        launch_convert_c128_c64(vj, x32_, n, s_, lanczos_done_flag(lw_));
+        heff<float>(kind, site, x32_, y32_, chil, chir);
+        lanczos_iteration_tail<float>(y32_, n, j, tau, p_.krylov_tol, lw_, s_);
+      }
+    }
+    QCA_HIP(hipMemcpyAsync(ctrl_host_, lw_.ctrl, sizeof(LanczosCtrl), hipMemcpyDeviceToHost, s_));
+    sync();
+    if (ctrl_host_->done) break;
+    QCA_CHECK(j < lw_.max_m - 2, "Krylov exponential did not converge");
+    chunk = 2;
+  }
+  const int m = ctrl_host_->m;
+  kpred_[kind] = std::max(2, m);
+  if (ctrl_host_->hit_limit) stats_.lanczos_limit_hits++;
+  lanczos_finish(n, x, lw_, s_);
+  stats_.matvecs += m;
+  if (kind == 0) stats_.matvec_2site += m;
+  else if (kind == 1) stats_.matvec_1site += m;
+  else stats_.matvec_bond += m;
+  const int D = kind == 2 ? Dl_[site] : std::max(Dl_[site], kind == 0 ? Dr_[site + 1] : Dr_[site]);
+  stats_.flops += m * heff_flops(kind, chil, chir, D);
+}
+
+void Engine::evolve(int kind, int site, c128* x, long n, int chil, int chir, double delta) {
+  if (p_.precision == PREC_MIXED) lanczos_run<float>(kind, site, x, n, chil, chir, delta);
+  else lanczos_run<double>(kind, site, x, n, chil, chir, delta);
+}
+
+// ---------------------------------------------------------------- environments
+// L[site][b][w'][b*] = sum A[p][a][b] W[p][p'][w][w'] conj(A[p'][a*][b*]) L[site-1][a][w][a*]
+// R[site][a][w][a*] = sum A[p][a][b] W[p][p'][w][w'] conj(A[p'][a*][b*]) R[site+1][b][w'][b*]
+template <typename T>
+void Engine::env_update_t(bool left, int site) {
+  const int chil = chi_[site], chir = chi_[site + 1];
+  const long lr = (long)chil * chir;
+  const cplx<T>* A;
+  if constexpr (std::is_same<T, double>::value) {
+    A = A_[site];
+  } else {
+    launch_convert_c128_c64(A_[site], a32_, 2 * lr, s_);
+    A = a32_;
+  }
+  const WSparsePair& wp = left ? Wf_[site] : Wb_[site];
+  const WSparseDev<T>* W;
+  if constexpr (std::is_same<T, double>::value) W = &wp.d64; else W = &wp.d32;
+  if (left)
+    env_update<T>(true, A, Lenv<cplx<T>>(site - 1), Lenv<cplx<T>>(site), W, Dl_[site], Dr_[site],
+                  chil, chir, ws_for<T>(), s_);
+  else
+    env_update<T>(false, A, Renv<cplx<T>>(site + 1), Renv<cplx<T>>(site), W, Dr_[site], Dl_[site],
+                  chil, chir, ws_for<T>(), s_);
+}
+
+void Engine::update_env(bool left, int site) {
+  if (p_.precision == PREC_MIXED) env_update_t<float>(left, site);
+  else env_update_t<double>(left, site);
+  const int chil = chi_[site], chir = chi_[site + 1];
+  const int D = std::max(Dl_[site], Dr_[site]);
+  stats_.flops += heff_flops(1, chil, chir, D);
+}
+
+// ---------------------------------------------------------------- 2-site split
+void Engine::merge_theta(int s, int chil, int chim, int chir) {
+  GemmDesc g = gemm_desc(chil, chir, chim);
+  g.NB1 = 2; g.NB2 = 2;
+  g.a_m = chim; g.a_k2 = 1; g.a_b1 = (long)chil * chim;
+  g.b_k2 = chir; g.b_n = 1; g.b_b2 = (long)chim * chir;
+  g.c_m = chir; g.c_n = 1; g.c_b1 = 2L * chil * chir; g.c_b2 = (long)chil * chir;
+  gemm<double>(g, A_[s], A_[s + 1], theta_);
+}
+
+int Engine::split_theta(int s, bool right_sweep, int chil, int chir) {
+  const int rows = right_sweep ? 2 * chil : 2 * chir;
+  const int cols = right_sweep ? 2 * chir : 2 * chil;
+  const int rank_cap = std::min(2 * chil, 2 * chir);
+  launch_theta_to_svd(theta_, svdX_, chil, chir, right_sweep ? 0 : 1, s_);
+  launch_jacobi_svd(svdX_, rows, cols, sigma_, perm_, svdres_, p_.chi_max, p_.eps, rank_cap, s_);
+  QCA_HIP(hipMemcpyAsync(svdres_host_, svdres_, sizeof(SvdResult), hipMemcpyDeviceToHost, s_));
+  sync();
+  const int k = svdres_host_->k;
+  stats_.svd_sweeps += svdres_host_->nsweeps;
+  QCA_CHECK(k >= 1, "SVD truncation produced an empty bond");
+  QCA_CHECK(k <= cap_[s + 1], "truncation rank above bond capacity");
+  const double inv = 1.0 / svdres_host_->norm_kept;
+  if (right_sweep) {
+    // A[s][p][a][k] = U[(p,a)][k]
+    launch_gather_isometry(svdX_, perm_, sigma_, A_[s], rows, chil, k, 0, s_);
+    // A[s+1][p2][k][c] = sum_{(p1,a)} conj(A[s][(p1,a)][k]) Theta[p1][p2][a][c] / |s_k|
+    GemmDesc g = gemm_desc(k, chir, 2 * chil);
+    g.K2 = chil;
+    g.NB2 = 2;
+    g.a_m = 1; g.a_k1 = (long)chil * k; g.a_k2 = k; g.conj_a = 1;
+    g.b_k1 = 2L * chil * chir; g.b_k2 = chir; g.b_n = 1; g.b_b2 = (long)chil * chir;
+    g.c_m = chir; g.c_n = 1; g.c_b2 = (long)k * chir;
+    g.alpha_re = inv;
+    gemm<double>(g, A_[s], theta_, A_[s + 1]);
+  } else {
+    // A[s+1][p2][k][c] = conj(V[(p2,c)][k])
+    launch_gather_isometry(svdX_, perm_, sigma_, A_[s + 1], rows, chir, k, 1, s_);
+    // A[s][p1][a][k] = sum_{(p2,c)} Theta[p1][p2][a][c] conj(A[s+1][p2][k][c]) / |s_k|
+    GemmDesc g = gemm_desc(chil, k, 2 * chir);
+    g.K2 = chir;
+    g.NB2 = 2;
+    g.a_m = chir; g.a_k1 = (long)chil * chir; g.a_k2 = 1; g.a_b2 = 2L * chil * chir;
+    g.b_k1 = (long)k * chir; g.b_k2 = 1; g.b_n = chir; g.conj_b = 1;
+    g.c_m = k; g.c_n = 1; g.c_b2 = (long)chil * k;
+    g.alpha_re = inv;
+    gemm<double>(g, theta_, A_[s + 1], A_[s]);
+  }
+  chi_[s + 1] = k;
+  return k;
+}
+
+void Engine::sweep_right_2() {
+  const double h = p_.dt / 2;
+  for (int s = 0; s + 1 < N_; ++s) {
+    const int chil = chi_[s], chim = chi_[s + 1], chir = chi_[s + 2];
+    merge_theta(s, chil, chim, chir);
+    evolve(0, s, theta_, 4L * chil * chir, chil, chir, h);
+    const int k = split_theta(s, true, chil, chir);
+    stats_.site_updates++;
+    if (s < N_ - 2) {
+      update_env(true, s);
+      evolve(1, s + 1, A_[s + 1], 2L * k * chir, k, chir, -h);
+    }
+  }
+}
+
+void Engine::sweep_left_2() {
+  const double h = p_.dt / 2;
+  for (int s = N_ - 1; s >= 1; --s) {
+    const int chil = chi_[s - 1], chim = chi_[s], chir = chi_[s + 1];
+    merge_theta(s - 1, chil, chim, chir);
+    evolve(0, s - 1, theta_, 4L * chil * chir, chil, chir, h);
+    const int k = split_theta(s - 1, false, chil, chir);
+    stats_.site_updates++;
+    if (s > 1) {
+      update_env(false, s);
+      evolve(1, s - 1, A_[s - 1], 2L * chil * k, chil, k, -h);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- QR shifts
+// Left QR of A[site] as (2 chil x chir): A[site] <- Q[:, :target] (zero padded),
+// C <- R[:target, :chir].  (reference mps.py:83-88, tdvp.py:74-78)
+void Engine::qr_left_site(int site, int target, c128* C, int& crow, int& ccol, bool) {
+  const int chil = chi_[site], chir = chi_[site + 1];
+  const int rows = 2 * chil, cols = chir;
+  QCA_CHECK(target <= cap_[site + 1], "target bond above capacity");
+  launch_to_colmajor(A_[site], svdX_, rows, cols, s_);
+  const int qcols = std::min(target, rows);
+  launch_householder_qr(svdX_, rows, cols, qrTau_, qrQ_, qcols, s_);
+  launch_qr_gather_left(qrQ_, svdX_, A_[site], C, rows, cols, qcols, target, s_);
+  crow = target;
+  ccol = chir;
+}
+
+// Right QR of A[site]^T as (2 chir x chil): A[site] <- Q^T[:target] (2, target, chir),
+// C <- R^T[:, :target] (chil x target).  (reference mps.py:90-98, tdvp.py:95-99)
+void Engine::qr_right_site(int site, int target, c128* C, int& crow, int& ccol, bool) {
+  const int chil = chi_[site], chir = chi_[site + 1];
+  const int rows = 2 * chir, cols = chil;
+  QCA_CHECK(target <= cap_[site], "target bond above capacity");
+  launch_site_to_rightqr(A_[site], svdX_, chil, chir, s_);
+  const int qcols = std::min(target, rows);
+  launch_householder_qr(svdX_, rows, cols, qrTau_, qrQ_, qcols, s_);
+  launch_qr_gather_right(qrQ_, svdX_, A_[site], C, chir, cols, qcols, target, s_);
+  crow = chil;
+  ccol = target;
+}
+
+// A[site] (2, ccol, chi_r) <- C (crow x ccol) * A[site]  => (2, crow, chi_r)
+void Engine::absorb_left(int site, const c128* C, int crow, int ccol) {
+  const int chir = chi_[site + 1];
+  GemmDesc g = gemm_desc(crow, chir, ccol);
+  g.NB2 = 2;
+  g.a_m = ccol; g.a_k2 = 1;
+  g.b_k2 = chir; g.b_n = 1; g.b_b2 = (long)ccol * chir;
+  g.c_m = chir; g.c_n = 1; g.c_b2 = (long)crow * chir;
+  gemm<double>(g, C, A_[site], theta2_);
+  QCA_HIP(hipMemcpyAsync(A_[site], theta2_, sizeof(c128) * 2L * crow * chir,
+                         hipMemcpyDeviceToDevice, s_));
+}
+
+// A[site] (2, chi_l, crow) <- A[site] * C (crow x ccol)  => (2, chi_l, ccol)
+void Engine::absorb_right(int site, const c128* C, int crow, int ccol) {
+  const int chil = chi_[site];
+  GemmDesc g = gemm_desc(2 * chil, ccol, crow);
+  g.a_m = crow; g.a_k2 = 1;
+  g.b_k2 = ccol; g.b_n = 1;
+  g.c_m = ccol; g.c_n = 1;
+  gemm<double>(g, A_[site], C, theta2_);
+  QCA_HIP(hipMemcpyAsync(A_[site], theta2_, sizeof(c128) * 2L * chil * ccol,
+                         hipMemcpyDeviceToDevice, s_));
+}
+
+// make_site_canonical(N-1) then make_site_canonical(0) with reduced QRs whose
+// factors are cut or zero-padded to keep every bond dimension
+// (reference tdvp.py:24-26, mps.py:146-192).
+void Engine::canonicalize_initial() {
+  int cr, cc;
+  for (int j = 0; j + 1 < N_; ++j) {
+    qr_left_site(j, chi_[j + 1], cbond_, cr, cc, false);
+    absorb_left(j + 1, cbond_, cr, cc);
+  }
+  for (int j = N_ - 1; j >= 1; --j) {
+    qr_right_site(j, chi_[j], cbond_, cr, cc, false);
+    absorb_right(j - 1, cbond_, cr, cc);
+  }
+}
+
+void Engine::sweep_right_1() {
+  const double h = p_.dt / 2;
+  for (int site = 0; site < N_; ++site) {
+    const int chil = chi_[site], chir = chi_[site + 1];
+    evolve(1, site, A_[site], 2L * chil * chir, chil, chir, h);
+    stats_.site_updates++;
+    if (site == N_ - 1) continue;
+    const int tgt = target_[site + 1];
+    int cr, cc;
+    qr_left_site(site, tgt, cbond_, cr, cc, true);
+    chi_[site + 1] = tgt;  // A[site] is (2, chil, tgt); A[site+1] still has left dim chir
+    update_env(true, site);
+    evolve(2, site + 1, cbond_, (long)cr * cc, cr, cc, -h);
+    // absorb: A[site+1] (2, chir, chi2) -> (2, tgt, chi2)
+    const int chi2 = chi_[site + 2];
+    GemmDesc g = gemm_desc(cr, chi2, cc);
+    g.NB2 = 2;
+    g.a_m = cc; g.a_k2 = 1;
+    g.b_k2 = chi2; g.b_n = 1; g.b_b2 = (long)cc * chi2;
+    g.c_m = chi2; g.c_n = 1; g.c_b2 = (long)cr * chi2;
+    gemm<double>(g, cbond_, A_[site + 1], theta2_);
+    QCA_HIP(hipMemcpyAsync(A_[site + 1], theta2_, sizeof(c128) * 2L * cr * chi2,
+                           hipMemcpyDeviceToDevice, s_));
+  }
+}
+
+void Engine::sweep_left_1() {
+  const double h = p_.dt / 2;
+  for (int site = N_ - 1; site >= 0; --site) {
+    const int chil = chi_[site], chir = chi_[site + 1];
+    evolve(1, site, A_[site], 2L * chil * chir, chil, chir, h);
+    stats_.site_updates++;
+    if (site == 0) continue;
+    const int tgt = target_[site];
+    int cr, cc;
+    qr_right_site(site, tgt, cbond_, cr, cc, true);
+    chi_[site] = tgt;  // A[site] is (2, tgt, chir); A[site-1] still has right dim chil
+    update_env(false, site);
+    evolve(2, site, cbond_, (long)cr * cc, cr, cc, -h);
+    const int chi0 = chi_[site - 1];
+    GemmDesc g = gemm_desc(2 * chi0, cc, cr);
+    g.a_m = cr; g.a_k2 = 1;
+    g.b_k2 = cc; g.b_n = 1;
+    g.c_m = cc; g.c_n = 1;
+    gemm<double>(g, A_[site - 1], cbond_, theta2_);
+    QCA_HIP(hipMemcpyAsync(A_[site - 1], theta2_, sizeof(c128) * 2L * chi0 * cc,
+                           hipMemcpyDeviceToDevice, s_));
+  }
+}
+
+void Engine::step() {
+  if (p_.algorithm == 2) {
+    sweep_right_2();
+    sweep_left_2();
+  } else {
+    sweep_right_1();
+    sweep_left_1();
+  }
+  sync();
+}
+
+// ---------------------------------------------------------------- measurement
+// rho_i[p][q] = sum E[x][y] A[p][x][b] conj(A[q][y][b]) with the left transfer
+// E_i = sum_p A_i^T E_{i-1} conj(A_i); equals the reference's centre-site rho
+// (mps.py:100-140) because sites 1..N-1 are right-isometric.
+void Engine::measure(c128* rho_host) {
+  c128 one{1.0, 0.0};
+  QCA_HIP(hipMemcpyAsync(measE_, &one, sizeof(one), hipMemcpyHostToDevice, s_));
+  for (int site = 0; site < N_; ++site) {
+    const int chil = chi_[site], chir = chi_[site + 1];
+    const long lr = (long)chil * chir;
+    // T[p][x][b] = sum_y E[x][y] conj(A[p][y][b])
+    GemmDesc g = gemm_desc(chil, chir, chil);
+    g.NB2 = 2;
+    g.a_m = chil; g.a_k2 = 1;
+    g.b_k2 = chir; g.b_n = 1; g.b_b2 = lr; g.conj_b = 1;
+    g.c_m = chir; g.c_n = 1; g.c_b2 = lr;
+    gemm<double>(g, measE_, A_[site], measT_);
+    launch_rho(A_[site], measT_, lr, rho_ + 4L * site, s_);
+    if (site + 1 < N_) {
+      // E[b][c] = sum_{(p,x)} A[(p,x)][b] T[(p,x)][c]
+      GemmDesc h = gemm_desc(chir, chir, 2 * chil);
+      h.a_m = 1; h.a_k2 = chir;
+      h.b_k2 = chir; h.b_n = 1;
+      h.c_m = chir; h.c_n = 1;
+      gemm<double>(h, A_[site], measT_, measE_);
+    }
+  }
+  QCA_HIP(hipMemcpyAsync(rho_host, rho_, sizeof(c128) * 4L * N_, hipMemcpyDeviceToHost, s_));
+  sync();
+}
+
+void Engine::get_site(int i, c128* host) const {
+  QCA_CHECK(i >= 0 && i < N_, "site index out of range");
+  QCA_HIP(hipMemcpyAsync(host, A_[i], sizeof(c128) * 2L * chi_[i] * chi_[i + 1],
+                         hipMemcpyDeviceToHost, s_));
+  QCA_HIP(hipStreamSynchronize(s_));
+}
+
+}  // namespace qca

@@ -1,0 +1,268 @@
+// Strided, batched complex GEMM on CDNA4 matrix cores (gfx950).
+//
+// One kernel body serves the plain path (complex128 on v_mfma_f64_16x16x4f64)
+// and the mixed path (complex64 on v_mfma_f32_16x16x4f32).  A complex product
+// is four real MFMAs on planar (re, im) LDS tiles:
+//   Cre += Are*Bre - Aim*Bim,   Cim += Are*Bim + Aim*Bre.
+// Tile 64x64x16, 256 threads = 4 wave64s in a 2x2 grid, each wave owning a
+// 32x32 block = 2x2 MFMA 16x16 tiles.  Operands are read with generic two-level
+// strides (see GemmDesc) so that every TDVP contraction is a single launch with
+// no permutation copies; the load loop walks the unit-stride index so global
+// reads stay coalesced.  Launches with too few output tiles to fill 256 CUs
+// split K and reduce the partial tiles in a second, deterministic pass.
+#include "qca_common.hpp"
+
+namespace qca {
+namespace {
+
+constexpr int BM = 64, BN = 64, BK = 16, PAD = 1;
+
+template <typename T>
+struct Mfma;
+// Both instructions take A[i][k] from lane i + 16k and B[k][j] from lane
+// j + 16k, and return column j = lane % 16, but they differ in which row the
+// four accumulator registers hold (measured on gfx950):
+//   f64 16x16x4: register r of lane l holds row (l / 16) + 4 r
+//   f32 16x16x4: register r of lane l holds row 4 (l / 16) + r
+template <>
+struct Mfma<double> {
+  typedef double acc_t __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc_t mac(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <>
+struct Mfma<float> {
+  typedef float acc_t __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc_t mac(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+template <typename T, bool A_MCONT, bool B_NCONT>
+__global__ __launch_bounds__(256) void gemm_kernel(GemmDesc d, const cplx<T>* __restrict__ A,
+                                                   const cplx<T>* __restrict__ B,
+                                                   cplx<T>* __restrict__ C,
+                                                   cplx<T>* __restrict__ ws, int splits,
+                                                   int k_per_split) {
+  using acc_t = typename Mfma<T>::acc_t;
+  if (d.skip && *d.skip) return;
+  __shared__ T As_re[BK][BM + PAD], As_im[BK][BM + PAD];
+  __shared__ T Bs_re[BK][BN + PAD], Bs_im[BK][BN + PAD];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = (wave >> 1) * 32;
+  const int wn = (wave & 1) * 32;
+  const int m0 = blockIdx.y * BM;
+  const int n0 = blockIdx.x * BN;
+  const int zb = blockIdx.z / splits;
+  const int split = blockIdx.z - zb * splits;
+  const int b1 = zb / d.NB2;
+  const int b2 = zb - b1 * d.NB2;
+
+  const cplx<T>* Ab = A + b1 * d.a_b1 + b2 * d.a_b2;
+  const cplx<T>* Bb = B + b1 * d.b_b1 + b2 * d.b_b2;
+
+  const int k_begin = split * k_per_split;
+  const int k_end = min(d.K, k_begin + k_per_split);
+
+  acc_t acc_re[2][2], acc_im[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      acc_re[i][j] = acc_t{0, 0, 0, 0};
+      acc_im[i][j] = acc_t{0, 0, 0, 0};
+    }
+
+  const T sa = d.conj_a ? T(-1) : T(1);
+  const T sb = d.conj_b ? T(-1) : T(1);
+
+  for (int k0 = k_begin; k0 < k_end; k0 += BK) {
+    // ---- stage A (BM x BK) and B (BK x BN) into LDS, split into re/im planes
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int ml, kl;
+      if (A_MCONT) {
+        ml = tid & 63;
+        kl = (tid >> 6) + 4 * r;
+      } else {
+        kl = tid & 15;
+        ml = (tid >> 4) + 16 * r;
+      }
+      const int m = m0 + ml, k = k0 + kl;
+      cplx<T> v{T(0), T(0)};
+      if (m < d.M && k < k_end) {
+        const int k1 = k / d.K2, k2 = k - k1 * d.K2;
+        v = Ab[m * d.a_m + k1 * d.a_k1 + k2 * d.a_k2];
+      }
+      As_re[kl][ml] = v.re;
+      As_im[kl][ml] = sa * v.im;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int nl, kl;
+      if (B_NCONT) {
+        nl = tid & 63;
+        kl = (tid >> 6) + 4 * r;
+      } else {
+        kl = tid & 15;
+        nl = (tid >> 4) + 16 * r;
+      }
+      const int n = n0 + nl, k = k0 + kl;
+      cplx<T> v{T(0), T(0)};
+      if (n < d.N && k < k_end) {
+        const int k1 = k / d.K2, k2 = k - k1 * d.K2;
+        v = Bb[k1 * d.b_k1 + k2 * d.b_k2 + n * d.b_n];
+      }
+      Bs_re[kl][nl] = v.re;
+      Bs_im[kl][nl] = sb * v.im;
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int kr = kk + (lane >> 4);
+      const int c = lane & 15;
+      T ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        ar[i] = As_re[kr][wm + 16 * i + c];
+        ai[i] = As_im[kr][wm + 16 * i + c];
+        br[i] = Bs_re[kr][wn + 16 * i + c];
+        bi[i] = Bs_im[kr][wn + 16 * i + c];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          acc_re[i][j] = Mfma<T>::mac(ar[i], br[j], acc_re[i][j]);
+          acc_re[i][j] = Mfma<T>::mac(-ai[i], bi[j], acc_re[i][j]);
+          acc_im[i][j] = Mfma<T>::mac(ar[i], bi[j], acc_im[i][j]);
+          acc_im[i][j] = Mfma<T>::mac(ai[i], br[j], acc_im[i][j]);
+        }
+    }
+    __syncthreads();
+  }
+
+  // ---- epilogue: column lane%16, rows per Mfma<T>::row of each 16x16 tile
+  const int col = lane & 15;
+  const T al_re = T(d.alpha_re), al_im = T(d.alpha_im);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm + 16 * i + Mfma<T>::row(lane, r);
+        const int n = n0 + wn + 16 * j + col;
+        if (m >= d.M || n >= d.N) continue;
+        const T xr = acc_re[i][j][r], xi = acc_im[i][j][r];
+        if (splits > 1) {
+          ws[((long)blockIdx.z * d.M + m) * d.N + n] = cplx<T>{xr, xi};
+        } else {
+          cplx<T>* cp = C + b1 * d.c_b1 + b2 * d.c_b2 + m * d.c_m + n * d.c_n;
+          cplx<T> out{al_re * xr - al_im * xi, al_re * xi + al_im * xr};
+          if (d.accumulate) {
+            out.re += cp->re;
+            out.im += cp->im;
+          }
+          *cp = out;
+        }
+      }
+}
+
+// Sum the split-K partial tiles in split order (deterministic) and apply
+// alpha/beta while scattering into C's strides.
+template <typename T>
+__global__ void splitk_reduce(GemmDesc d, const cplx<T>* __restrict__ ws, cplx<T>* __restrict__ C,
+                              int splits, int batch) {
+  if (d.skip && *d.skip) return;
+  const long mn = (long)d.M * d.N;
+  const long total = mn * batch;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const int zb = (int)(idx / mn);
+    const long r = idx - zb * mn;
+    const int m = (int)(r / d.N), n = (int)(r - (long)m * d.N);
+    T xr = 0, xi = 0;
+    for (int s = 0; s < splits; ++s) {
+      const cplx<T> v = ws[((long)(zb * splits + s) * d.M + m) * d.N + n];
+      xr += v.re;
+      xi += v.im;
+    }
+    const int b1 = zb / d.NB2, b2 = zb - b1 * d.NB2;
+    cplx<T>* cp = C + b1 * d.c_b1 + b2 * d.c_b2 + m * d.c_m + n * d.c_n;
+    const T al_re = T(d.alpha_re), al_im = T(d.alpha_im);
+    cplx<T> out{al_re * xr - al_im * xi, al_re * xi + al_im * xr};
+    if (d.accumulate) {
+      out.re += cp->re;
+      out.im += cp->im;
+    }
+    *cp = out;
+  }
+}
+
+template <typename T>
+void gemm_launch(const GemmDesc& d, const cplx<T>* A, const cplx<T>* B, cplx<T>* C,
+                 hipStream_t s, cplx<T>* ws, long ws_elems) {
+  QCA_CHECK(d.M >= 0 && d.N >= 0 && d.K >= 0, "negative GEMM extent");
+  QCA_CHECK(d.K2 >= 1 && (d.K == 0 || d.K % d.K2 == 0), "K2 must divide K");
+  QCA_CHECK(d.NB1 >= 1 && d.NB2 >= 1, "empty batch");
+  const int batch = d.NB1 * d.NB2;
+  if (d.M == 0 || d.N == 0) return;
+  const int tm = ceil_div(d.M, BM), tn = ceil_div(d.N, BN);
+  const long tiles = (long)tm * tn * batch;
+  // split K when the output tiles alone cannot occupy the chip
+  int splits = 1;
+  if (tiles < 192 && d.K >= 4 * BK) {
+    splits = (int)std::min<long>((384 + tiles - 1) / tiles, d.K / (2 * BK));
+    while (splits > 1 && (long)splits * batch * d.M * d.N > ws_elems) --splits;
+    if (splits < 1) splits = 1;
+  }
+  int kps = d.K;
+  if (splits > 1) {
+    kps = ceil_div(ceil_div(d.K, splits), BK) * BK;
+    splits = ceil_div(d.K, kps);
+  }
+  if (d.K == 0) {
+    // C = beta*C: only zeroing is needed when not accumulating
+    splits = 1;
+    kps = 0;
+  }
+  QCA_CHECK((long)tiles * splits < (1L << 31), "GEMM grid too large");
+  QCA_CHECK(batch * splits <= 65535, "GEMM batch*splits exceeds grid z limit");
+  dim3 grid(tn, tm, batch * splits);
+  const bool am = (d.a_m == 1), bn = (d.b_n == 1);
+#define QCA_GEMM_L(AM, BNC) \
+  hipLaunchKernelGGL((gemm_kernel<T, AM, BNC>), grid, dim3(256), 0, s, d, A, B, C, ws, splits, kps)
+  if (am && bn) QCA_GEMM_L(true, true);
+  else if (am) QCA_GEMM_L(true, false);
+  else if (bn) QCA_GEMM_L(false, true);
+  else QCA_GEMM_L(false, false);
+#undef QCA_GEMM_L
+  QCA_HIP(hipGetLastError());
+  if (splits > 1) {
+    const long total = (long)d.M * d.N * batch;
+    const int blocks = (int)std::min<long>(ceil_div(total, 256), 4096);
+    hipLaunchKernelGGL(splitk_reduce<T>, dim3(blocks), dim3(256), 0, s, d, ws, C, splits, batch);
+    QCA_HIP(hipGetLastError());
+  }
+}
+
+}  // namespace
+
+void gemm_c128(const GemmDesc& d, const c128* A, const c128* B, c128* C, hipStream_t s,
+               c128* ws, long ws_elems) {
+  gemm_launch<double>(d, A, B, C, s, ws, ws_elems);
+}
+void gemm_c64(const GemmDesc& d, const c64* A, const c64* B, c64* C, hipStream_t s, c64* ws,
+              long ws_elems) {
+  gemm_launch<float>(d, A, B, C, s, ws, ws_elems);
+}
+
+}  // namespace qca

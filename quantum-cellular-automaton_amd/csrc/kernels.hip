@@ -1,0 +1,697 @@
+// Non-GEMM device kernels of the TDVP path (gfx950).
+//
+//  * wmix      -- sparse MPO mixing between the two GEMMs of every contraction
+//  * permute   -- Theta <-> SVD matrix layouts, gathers of singular vectors
+//  * lanczos   -- FP64 Krylov exp(-i tau H) v with CGS2 reorthogonalisation;
+//                 the tridiagonal exponential runs on one wave (Taylor with
+//                 scaling) so no host round trip is needed per iteration
+//  * measure   -- <n_i> and the single-site density matrix from the transfer
+//  * exact     -- closed-form 2^N rule-Hamiltonian application
+#include "kernels.hpp"
+
+namespace qca {
+namespace {
+
+// ------------------------------------------------------------------ helpers
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return cplx<T>{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cmulc(cplx<T> a, cplx<T> b) {  // conj(a) * b
+  return cplx<T>{a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block-wide sum of a double; blockDim.x must be a multiple of 64, <= 1024.
+__device__ double block_sum(double v, double* sh) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  const int nw = blockDim.x >> 6;
+  double r = 0;
+  if (threadIdx.x < 64) {
+    r = (l < nw) ? sh[l] : 0.0;
+    r = wave_sum(r);
+    if (l == 0) sh[0] = r;
+  }
+  __syncthreads();
+  r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// ------------------------------------------------------------------ wmix
+// U[q][xc][v] = sum_{i} Wsp[(q,v), i] * Tin[i][xc]
+// Input channels are staged through LDS so every input element is read from
+// memory once; the output rows [q][xc0..xc0+63][0..Dout) are written as one
+// contiguous run.
+constexpr int WMIX_XC = 64;
+constexpr int WMIX_DMAX = 16;
+
+template <typename T>
+__global__ __launch_bounds__(64) void wmix_kernel(const cplx<T>* __restrict__ Tin, long in_stride,
+                                                  long nxc, int nin, cplx<T>* __restrict__ U,
+                                                  int nq, int Dout, const int* __restrict__ rowptr,
+                                                  const int* __restrict__ col,
+                                                  const cplx<T>* __restrict__ val,
+                                                  const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* sin = reinterpret_cast<cplx<T>*>(smem_raw);        // [nin][64]
+  cplx<T>* sout = sin + (long)nin * WMIX_XC;                   // [64][Dout]
+  const int t = threadIdx.x;
+  const long xc0 = (long)blockIdx.x * WMIX_XC;
+  const long xc = xc0 + t;
+  const bool live = xc < nxc;
+  for (int i = 0; i < nin; ++i)
+    sin[i * WMIX_XC + t] = live ? Tin[(long)i * in_stride + xc] : cplx<T>{T(0), T(0)};
+  const int nrun = (int)min<long>(WMIX_XC, nxc - xc0);
+  for (int q = 0; q < nq; ++q) {
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < WMIX_DMAX; ++v) {
+      if (v < Dout) {
+        const int o = q * Dout + v;
+        T ar = 0, ai = 0;
+        for (int z = rowptr[o]; z < rowptr[o + 1]; ++z) {
+          const cplx<T> x = sin[col[z] * WMIX_XC + t];
+          const cplx<T> w = val[z];
+          ar += w.re * x.re - w.im * x.im;
+          ai += w.re * x.im + w.im * x.re;
+        }
+        sout[t * Dout + v] = cplx<T>{ar, ai};
+      }
+    }
+    __syncthreads();
+    cplx<T>* dst = U + ((long)q * nxc + xc0) * Dout;
+    for (int e = t; e < nrun * Dout; e += WMIX_XC) dst[e] = sout[e];
+  }
+}
+
+// ------------------------------------------------------------------ conversions
+template <typename S, typename D>
+__global__ void convert_kernel(const cplx<S>* __restrict__ in, cplx<D>* __restrict__ out, long n,
+                               const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i] = cplx<D>{D(in[i].re), D(in[i].im)};
+}
+
+// ------------------------------------------------------------------ permutes
+// Theta[p1][p2][a][c] -> column-major X of the SVD matrix M[(p1,a),(p2,c)].
+// mode 0: X[j][r] = M[r][j] (columns of M); mode 1: X[r][j] = conj(M[r][j])
+// (columns of M^dagger).
+__global__ void theta_to_svd(const c128* __restrict__ th, c128* __restrict__ X, int chil, int chir,
+                             int mode) {
+  const long m = 2L * chil, n = 2L * chir;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < m * n;
+       idx += (long)gridDim.x * blockDim.x) {
+    // idx enumerates Theta in memory order: p1, p2, a, c
+    const long c = idx % chir;
+    long t = idx / chir;
+    const long a = t % chil;
+    t /= chil;
+    const long p2 = t % 2, p1 = t / 2;
+    const long r = p1 * chil + a, j = p2 * chir + c;
+    c128 v = th[idx];
+    if (mode == 0) {
+      X[j * m + r] = v;
+    } else {
+      v.im = -v.im;
+      X[r * n + j] = v;
+    }
+  }
+}
+
+// A[p][a][k] = X[perm[k]][(p,a)] / sigma[k]   (mode 0, left isometry from U)
+// A[p][k][c] = conj(X[perm[k]][(p,c)]) / sigma[k]  (mode 1, right isometry from V)
+__global__ void gather_isometry(const c128* __restrict__ X, const int* __restrict__ perm,
+                                const double* __restrict__ sigma, c128* __restrict__ A, int rows,
+                                int chi_bond, int k, int mode) {
+  // rows = 2*chi_bond (length of each singular vector)
+  const long total = (long)rows * k;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const int kk = (int)(idx % k);
+    const int r = (int)(idx / k);
+    const double s = sigma[kk];
+    const double inv = s > 0 ? 1.0 / s : 0.0;
+    c128 v = X[(long)perm[kk] * rows + r];
+    const int p = r / chi_bond, a = r - p * chi_bond;
+    if (mode == 0) {
+      A[((long)p * chi_bond + a) * k + kk] = c128{v.re * inv, v.im * inv};
+    } else {
+      A[((long)p * k + kk) * chi_bond + a] = c128{v.re * inv, -v.im * inv};
+    }
+  }
+}
+
+// Row-major (rows x cols) -> column-major copy: out[c*rows + r] = in[r*cols + c]
+__global__ void to_colmajor(const c128* __restrict__ in, c128* __restrict__ out, int rows, int cols) {
+  const long n = (long)rows * cols;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const long r = e / cols, c = e - r * cols;
+    out[c * rows + r] = in[e];
+  }
+}
+
+// A[p][a][b] -> column-major X of A^T reshaped: X[a][(p,b)], rows = 2*chir
+__global__ void site_to_rightqr(const c128* __restrict__ A, c128* __restrict__ X, int chil, int chir) {
+  const long n = 2L * chil * chir;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const long b = e % chir, t = e / chir;
+    const long a = t % chil, p = t / chil;
+    X[a * (2L * chir) + p * chir + b] = A[e];
+  }
+}
+
+// Left QR results: Anew[(p,a)][k] = Q[k][(p,a)] (k < qcols, else 0) for k < target;
+// C[k][b] = R[k][b] (upper triangle of X, zero below / beyond min(rows, cols)).
+__global__ void qr_gather_left(const c128* __restrict__ Q, const c128* __restrict__ X, c128* __restrict__ Anew,
+                               c128* __restrict__ C, int rows, int cols, int qcols, int target) {
+  const long na = (long)rows * target, nc = (long)target * cols;
+  const int kmin = min(rows, cols);
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < na + nc; e += (long)gridDim.x * blockDim.x) {
+    if (e < na) {
+      const long r = e / target, k = e - r * target;
+      Anew[e] = (k < qcols) ? Q[k * rows + r] : c128{0.0, 0.0};
+    } else {
+      const long f = e - na;
+      const long k = f / cols, b = f - k * cols;
+      C[f] = (k <= b && k < kmin) ? X[b * rows + k] : c128{0.0, 0.0};
+    }
+  }
+}
+
+// Right QR results: Anew[p][k][b] = Q[k][(p,b)] (k < qcols, else 0), rows = 2*chir;
+// C[a][k] = R[k][a] for k < target.
+__global__ void qr_gather_right(const c128* __restrict__ Q, const c128* __restrict__ X, c128* __restrict__ Anew,
+                                c128* __restrict__ C, int chir, int cols, int qcols, int target) {
+  const int rows = 2 * chir;
+  const long na = 2L * target * chir, nc = (long)cols * target;
+  const int kmin = min(rows, cols);
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < na + nc; e += (long)gridDim.x * blockDim.x) {
+    if (e < na) {
+      const long b = e % chir, t = e / chir;
+      const long k = t % target, p = t / target;
+      Anew[e] = (k < qcols) ? Q[k * rows + p * chir + b] : c128{0.0, 0.0};
+    } else {
+      const long f = e - na;
+      const long a = f / target, k = f - a * target;
+      C[f] = (k <= a && k < kmin) ? X[a * rows + k] : c128{0.0, 0.0};
+    }
+  }
+}
+
+// ------------------------------------------------------------------ lanczos
+constexpr int LZ_BLOCK = 256;
+constexpr int LZ_EPT = 4;                     // vector elements per thread
+constexpr int LZ_ELEMS = LZ_BLOCK * LZ_EPT;   // elements per block in the partial reductions
+
+__global__ void lz_norm_partials(const c128* __restrict__ x, long n, double* __restrict__ part) {
+  __shared__ double sh[16];
+  double acc = 0;
+  const long base = (long)blockIdx.x * LZ_ELEMS;
+  for (long i = base + threadIdx.x; i < min(n, base + LZ_ELEMS); i += blockDim.x)
+    acc += x[i].re * x[i].re + x[i].im * x[i].im;
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+__global__ void lz_init_finalize(LanczosCtrl* ctrl, const double* __restrict__ part, int nblk) {
+  if (threadIdx.x != 0) return;
+  double s = 0;
+  for (int b = 0; b < nblk; ++b) s += part[b];
+  ctrl->beta0 = sqrt(s);
+  ctrl->done = (s == 0.0) ? 1 : 0;
+  ctrl->m = 0;
+  ctrl->err = 0;
+  ctrl->zero = (s == 0.0) ? 1 : 0;
+  ctrl->hit_limit = 0;
+}
+
+__global__ void lz_scale_first(const c128* __restrict__ x, c128* __restrict__ v0, long n,
+                               const LanczosCtrl* ctrl) {
+  const double b = ctrl->beta0;
+  const double inv = b > 0 ? 1.0 / b : 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    v0[i] = c128{x[i].re * inv, x[i].im * inv};
+}
+
+// Partial projections c_i = <V_i, w> over this block's chunk.  Each thread
+// keeps its LZ_EPT elements of w in registers across the i loop.
+template <typename TW>
+__global__ __launch_bounds__(LZ_BLOCK) void lz_project(const LanczosCtrl* ctrl,
+                                                       const cplx<TW>* __restrict__ w,
+                                                       const c128* __restrict__ V, long n, int j,
+                                                       c128* __restrict__ part) {
+  if (ctrl->done) return;
+  __shared__ double sh[16];
+  const long base = (long)blockIdx.x * LZ_ELEMS + threadIdx.x;
+  double wr[LZ_EPT], wi[LZ_EPT];
+#pragma unroll
+  for (int r = 0; r < LZ_EPT; ++r) {
+    const long e = base + (long)r * LZ_BLOCK;
+    wr[r] = e < n ? double(w[e].re) : 0.0;
+    wi[r] = e < n ? double(w[e].im) : 0.0;
+  }
+  for (int i = 0; i <= j; ++i) {
+    const c128* vi = V + (long)i * n;
+    double ar = 0, ai = 0;
+#pragma unroll
+    for (int r = 0; r < LZ_EPT; ++r) {
+      const long e = base + (long)r * LZ_BLOCK;
+      if (e < n) {
+        const c128 a = vi[e];
+        ar += a.re * wr[r] + a.im * wi[r];
+        ai += a.re * wi[r] - a.im * wr[r];
+      }
+    }
+    ar = block_sum(ar, sh);
+    ai = block_sum(ai, sh);
+    if (threadIdx.x == 0) part[(long)blockIdx.x * LZ_MAXK + i] = c128{ar, ai};
+  }
+}
+
+__global__ void lz_reduce_coeffs(const LanczosCtrl* ctrl, const c128* __restrict__ part, int nblk,
+                                 int j, c128* __restrict__ coef) {
+  if (ctrl->done) return;
+  const int i = threadIdx.x;
+  if (i > j) return;
+  double ar = 0, ai = 0;
+  for (int b = 0; b < nblk; ++b) {
+    ar += part[(long)b * LZ_MAXK + i].re;
+    ai += part[(long)b * LZ_MAXK + i].im;
+  }
+  coef[i] = c128{ar, ai};
+}
+
+// wout = w - sum_i coef_i V_i (promoting w to FP64), then partial projections
+// of wout (pass 1) or its partial squared norm (pass 2).
+template <typename TW>
+__global__ __launch_bounds__(LZ_BLOCK) void lz_subtract(const LanczosCtrl* ctrl,
+                                                        const cplx<TW>* __restrict__ w,
+                                                        c128* __restrict__ wout,
+                                                        const c128* __restrict__ V, long n, int j,
+                                                        const c128* __restrict__ coef,
+                                                        c128* __restrict__ part, int pass) {
+  if (ctrl->done) return;
+  __shared__ double sh[16];
+  __shared__ c128 cs[LZ_MAXK];
+  if (threadIdx.x <= j) cs[threadIdx.x] = coef[threadIdx.x];
+  __syncthreads();
+  const long base = (long)blockIdx.x * LZ_ELEMS + threadIdx.x;
+  double xr[LZ_EPT], xi[LZ_EPT];
+#pragma unroll
+  for (int r = 0; r < LZ_EPT; ++r) {
+    const long e = base + (long)r * LZ_BLOCK;
+    xr[r] = e < n ? double(w[e].re) : 0.0;
+    xi[r] = e < n ? double(w[e].im) : 0.0;
+  }
+  for (int i = 0; i <= j; ++i) {
+    const c128 c = cs[i];
+#pragma unroll
+    for (int r = 0; r < LZ_EPT; ++r) {
+      const long e = base + (long)r * LZ_BLOCK;
+      if (e < n) {
+        const c128 a = V[(long)i * n + e];
+        xr[r] -= c.re * a.re - c.im * a.im;
+        xi[r] -= c.re * a.im + c.im * a.re;
+      }
+    }
+  }
+  double nrm = 0;
+#pragma unroll
+  for (int r = 0; r < LZ_EPT; ++r) {
+    const long e = base + (long)r * LZ_BLOCK;
+    if (e < n) {
+      wout[e] = c128{xr[r], xi[r]};
+      nrm += xr[r] * xr[r] + xi[r] * xi[r];
+    }
+  }
+  if (pass == 2) {
+    nrm = block_sum(nrm, sh);
+    if (threadIdx.x == 0) part[(long)blockIdx.x * LZ_MAXK] = c128{nrm, 0};
+    return;
+  }
+  for (int i = 0; i <= j; ++i) {
+    const c128* vi = V + (long)i * n;
+    double ar = 0, ai = 0;
+#pragma unroll
+    for (int r = 0; r < LZ_EPT; ++r) {
+      const long e = base + (long)r * LZ_BLOCK;
+      if (e < n) {
+        const c128 a = vi[e];
+        ar += a.re * xr[r] + a.im * xi[r];
+        ai += a.re * xi[r] - a.im * xr[r];
+      }
+    }
+    ar = block_sum(ar, sh);
+    ai = block_sum(ai, sh);
+    if (threadIdx.x == 0) part[(long)blockIdx.x * LZ_MAXK + i] = c128{ar, ai};
+  }
+}
+
+// One wave: close iteration j.  alpha_j = Re(c_j + c'_j), beta_j = |w|;
+// z = exp(-i tau T_{j+1}) e_0 by scaled Taylor steps on the tridiagonal
+// (lane = row); converged when beta0*beta_j*|z_j| < tol*beta0.
+__global__ void lz_step(LanczosCtrl* ctrl, const c128* __restrict__ part_norm, int nblk,
+                        const c128* __restrict__ coef1, const c128* __restrict__ coef2, int j,
+                        double tau, double tol, int max_m) {
+  if (ctrl->done) return;
+  const int l = threadIdx.x;  // 64 lanes
+  double s = 0;
+  for (int b = l; b < nblk; b += 64) s += part_norm[(long)b * LZ_MAXK].re;
+  s = wave_sum(s);
+  const double beta = sqrt(s);
+  const double alpha_j = coef1[j].re + coef2[j].re;
+  if (l == 0) {
+    ctrl->alpha[j] = alpha_j;
+    ctrl->beta[j] = beta;
+  }
+  const int m = j + 1;
+  // alpha[0..j-1] and beta[0..j-1] were written by earlier launches
+  const double al = (l < j) ? ctrl->alpha[l] : ((l == j) ? alpha_j : 0.0);
+  const double bup = (l + 1 < m) ? ctrl->beta[l] : 0.0;     // T[l][l+1]
+  const double bdn = (l >= 1 && l < m) ? ctrl->beta[l - 1] : 0.0;  // T[l][l-1]
+  double rown = fabs(al) + fabs(bup) + fabs(bdn);
+  for (int o = 32; o > 0; o >>= 1) rown = fmax(rown, __shfl_xor(rown, o, 64));
+  const double h = fabs(tau) * rown;
+  // bounded even for non-finite input (h = NaN/inf gives one substep and a
+  // non-finite error estimate, which the host reports as non-convergence)
+  const int nsub = (h < 2048.0) ? max(1, (int)ceil(h / 0.5)) : 1;
+  const double dt = tau / nsub;
+  // z <- exp(-i dt T) z, nsub times;  multiplying by -i dt: (x+iy)*(-i dt) = dt*y - i dt*x
+  double zr = (l == 0) ? 1.0 : 0.0, zi = 0.0;
+  for (int sstep = 0; sstep < nsub; ++sstep) {
+    double ar = zr, ai = zi, tr = zr, ti = zi;
+    for (int k = 1; k < 60; ++k) {
+      const double upr = __shfl_down(tr, 1, 64), upi = __shfl_down(ti, 1, 64);
+      const double dnr = __shfl_up(tr, 1, 64), dni = __shfl_up(ti, 1, 64);
+      double yr = al * tr + bup * upr + bdn * dnr;
+      double yi = al * ti + bup * upi + bdn * dni;
+      if (l >= m) { yr = 0; yi = 0; }
+      const double nr = dt * yi / k, ni = -dt * yr / k;
+      tr = nr;
+      ti = ni;
+      ar += tr;
+      ai += ti;
+      double mag = fabs(tr) + fabs(ti);
+      for (int o = 32; o > 0; o >>= 1) mag = fmax(mag, __shfl_xor(mag, o, 64));
+      if (mag < 1e-18) break;
+    }
+    zr = ar;
+    zi = ai;
+  }
+  const double zlr = __shfl(zr, j, 64), zli = __shfl(zi, j, 64);
+  const double err = beta * sqrt(zlr * zlr + zli * zli);
+  const bool breakdown = beta <= 1e-13 * fmax(1.0, fabs(alpha_j));
+  const bool conv = breakdown || err < tol || m >= max_m - 1;
+  if (l < m) {
+    ctrl->z[l] = c128{zr, zi};
+  }
+  if (l == 0) {
+    ctrl->err = err;
+    if (conv) {
+      ctrl->m = m;
+      ctrl->done = 1;
+      ctrl->hit_limit = (!breakdown && err >= tol) ? 1 : 0;
+    }
+  }
+}
+
+__global__ void lz_scale_next(const LanczosCtrl* ctrl, const c128* __restrict__ w,
+                              c128* __restrict__ vnext, long n, int j) {
+  if (ctrl->done) return;
+  const double inv = 1.0 / ctrl->beta[j];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    vnext[i] = c128{w[i].re * inv, w[i].im * inv};
+}
+
+// y = beta0 * sum_{i<m} z_i V_i   (a zero input stays zero)
+__global__ void lz_combine(const LanczosCtrl* ctrl, const c128* __restrict__ V, long n,
+                           c128* __restrict__ y) {
+  __shared__ c128 zs[LZ_MAXK];
+  const int m = ctrl->m;
+  const double b0 = ctrl->beta0;
+  if (threadIdx.x < m) zs[threadIdx.x] = ctrl->z[threadIdx.x];
+  __syncthreads();
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    double r = 0, im = 0;
+    for (int i = 0; i < m; ++i) {
+      const c128 a = V[(long)i * n + e];
+      r += zs[i].re * a.re - zs[i].im * a.im;
+      im += zs[i].re * a.im + zs[i].im * a.re;
+    }
+    y[e] = c128{b0 * r, b0 * im};
+  }
+}
+
+// ------------------------------------------------------------------ measurement
+// rho[p][q] = sum_{x,b} A[p][x][b] * T[q][x][b], T[q][x][b] = sum_y E[x][y] conj(A[q][y][b])
+// One block; writes rho (2x2 complex) for one site.
+__global__ void rho_kernel(const c128* __restrict__ A, const c128* __restrict__ T, long nxb,
+                           c128* __restrict__ rho) {
+  __shared__ double sh[16];
+  for (int p = 0; p < 2; ++p)
+    for (int q = 0; q < 2; ++q) {
+      double ar = 0, ai = 0;
+      for (long e = threadIdx.x; e < nxb; e += blockDim.x) {
+        const c128 a = A[(long)p * nxb + e], t = T[(long)q * nxb + e];
+        ar += a.re * t.re - a.im * t.im;
+        ai += a.re * t.im + a.im * t.re;
+      }
+      ar = block_sum(ar, sh);
+      ai = block_sum(ai, sh);
+      if (threadIdx.x == 0) rho[p * 2 + q] = c128{ar, ai};
+    }
+}
+
+// ------------------------------------------------------------------ exact 2^N
+// (H psi)[x] = sum_i [popcount(x & nbr_i) in [lo, hi)] psi[x ^ bit_i]
+template <typename T>
+__global__ void exact_apply_kernel(const cplx<T>* __restrict__ psi, cplx<T>* __restrict__ out,
+                                   int ncells, int distance, int lo, int hi) {
+  const long dim = 1L << ncells;
+  for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < dim; x += (long)gridDim.x * blockDim.x) {
+    T ar = 0, ai = 0;
+    for (int i = 0; i < ncells; ++i) {
+      const int jlo = max(0, i - distance), jhi = min(ncells - 1, i + distance);
+      // neighbours j in [jlo, jhi], j != i; cell j is bit (N-1-j)
+      const int bhi = ncells - 1 - jlo, blo = ncells - 1 - jhi;
+      const unsigned long long span =
+          ((bhi - blo + 1) >= 64) ? ~0ULL : (((1ULL << (bhi - blo + 1)) - 1ULL) << blo);
+      const unsigned long long self = 1ULL << (ncells - 1 - i);
+      const int cnt = __popcll((unsigned long long)x & span & ~self);
+      if (cnt >= lo && cnt < hi) {
+        const cplx<T> v = psi[x ^ (long)self];
+        ar += v.re;
+        ai += v.im;
+      }
+    }
+    out[x] = cplx<T>{ar, ai};
+  }
+}
+
+// Single-site reduced density matrices of a 2^N state, two-stage reduction.
+// rho_i[p][q] = sum_{x: x_i = p} psi_x conj(psi_{x with x_i = q}); site i is bit N-1-i.
+// Stage 1: block (bx, i) writes partial (rho00, rho11, Re rho01, Im rho01).
+__global__ void exact_rho_partials(const c128* __restrict__ psi, int ncells, double* __restrict__ part) {
+  __shared__ double sh[16];
+  const int i = blockIdx.y;
+  const long dim = 1L << ncells;
+  const long bit = 1L << (ncells - 1 - i);
+  double d0 = 0, d1 = 0, orr = 0, oi = 0;
+  for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < dim; x += (long)gridDim.x * blockDim.x) {
+    const c128 a = psi[x];
+    const double n2 = a.re * a.re + a.im * a.im;
+    if (x & bit) {
+      d1 += n2;
+    } else {
+      d0 += n2;
+      const c128 b = psi[x | bit];  // conj(b) * a summed into rho01 = sum a conj(b)
+      orr += a.re * b.re + a.im * b.im;
+      oi += a.im * b.re - a.re * b.im;
+    }
+  }
+  d0 = block_sum(d0, sh);
+  d1 = block_sum(d1, sh);
+  orr = block_sum(orr, sh);
+  oi = block_sum(oi, sh);
+  if (threadIdx.x == 0) {
+    double* p = part + ((long)i * gridDim.x + blockIdx.x) * 4;
+    p[0] = d0; p[1] = d1; p[2] = orr; p[3] = oi;
+  }
+}
+
+__global__ void exact_rho_finalize(const double* __restrict__ part, int nbx, int ncells, c128* __restrict__ rho) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncells) return;
+  double d0 = 0, d1 = 0, orr = 0, oi = 0;
+  for (int b = 0; b < nbx; ++b) {
+    const double* p = part + ((long)i * nbx + b) * 4;
+    d0 += p[0]; d1 += p[1]; orr += p[2]; oi += p[3];
+  }
+  rho[4L * i + 0] = c128{d0, 0.0};
+  rho[4L * i + 1] = c128{orr, oi};
+  rho[4L * i + 2] = c128{orr, -oi};
+  rho[4L * i + 3] = c128{d1, 0.0};
+}
+
+int grid_for(long n, int block = 256, int maxb = 8192) {
+  return (int)std::max<long>(1, std::min<long>((n + block - 1) / block, maxb));
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+template <typename T>
+void launch_wmix(const cplx<T>* Tin, long in_stride, long nxc, int nin, cplx<T>* U, int nq,
+                 int Dout, const WSparseDev<T>& w, hipStream_t s, const int* skip) {
+  QCA_CHECK(Dout <= WMIX_DMAX, "MPO bond dimension above 16");
+  QCA_CHECK(nin <= 64, "too many input channels in MPO mixing");
+  if (nxc == 0) return;
+  const size_t shm = ((size_t)nin * WMIX_XC + (size_t)WMIX_XC * Dout) * sizeof(cplx<T>);
+  static bool attr_set = false;
+  if (!attr_set) {
+    QCA_HIP(hipFuncSetAttribute((const void*)wmix_kernel<T>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(wmix_kernel<T>, dim3(ceil_div(nxc, WMIX_XC)), dim3(WMIX_XC), shm, s, Tin,
+                     in_stride, nxc, nin, U, nq, Dout, w.rowptr, w.col, w.val, skip);
+  QCA_HIP(hipGetLastError());
+}
+template void launch_wmix<double>(const c128*, long, long, int, c128*, int, int,
+                                  const WSparseDev<double>&, hipStream_t, const int*);
+template void launch_wmix<float>(const c64*, long, long, int, c64*, int, int,
+                                 const WSparseDev<float>&, hipStream_t, const int*);
+
+void launch_convert_c128_c64(const c128* in, c64* out, long n, hipStream_t s, const int* skip) {
+  if (n == 0) return;
+  hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(n)), dim3(256), 0, s, in, out, n, skip);
+  QCA_HIP(hipGetLastError());
+}
+void launch_convert_c64_c128(const c64* in, c128* out, long n, hipStream_t s, const int* skip) {
+  if (n == 0) return;
+  hipLaunchKernelGGL((convert_kernel<float, double>), dim3(grid_for(n)), dim3(256), 0, s, in, out, n, skip);
+  QCA_HIP(hipGetLastError());
+}
+
+void launch_to_colmajor(const c128* in, c128* out, int rows, int cols, hipStream_t s) {
+  hipLaunchKernelGGL(to_colmajor, dim3(grid_for((long)rows * cols)), dim3(256), 0, s, in, out, rows, cols);
+  QCA_HIP(hipGetLastError());
+}
+void launch_site_to_rightqr(const c128* A, c128* X, int chil, int chir, hipStream_t s) {
+  hipLaunchKernelGGL(site_to_rightqr, dim3(grid_for(2L * chil * chir)), dim3(256), 0, s, A, X, chil, chir);
+  QCA_HIP(hipGetLastError());
+}
+void launch_qr_gather_left(const c128* Q, const c128* X, c128* Anew, c128* C, int rows, int cols,
+                           int qcols, int target, hipStream_t s) {
+  const long n = (long)rows * target + (long)target * cols;
+  hipLaunchKernelGGL(qr_gather_left, dim3(grid_for(n)), dim3(256), 0, s, Q, X, Anew, C, rows, cols, qcols, target);
+  QCA_HIP(hipGetLastError());
+}
+void launch_qr_gather_right(const c128* Q, const c128* X, c128* Anew, c128* C, int chir, int cols,
+                            int qcols, int target, hipStream_t s) {
+  const long n = 2L * target * chir + (long)cols * target;
+  hipLaunchKernelGGL(qr_gather_right, dim3(grid_for(n)), dim3(256), 0, s, Q, X, Anew, C, chir, cols, qcols, target);
+  QCA_HIP(hipGetLastError());
+}
+
+void launch_theta_to_svd(const c128* th, c128* X, int chil, int chir, int mode, hipStream_t s) {
+  const long n = 4L * chil * chir;
+  hipLaunchKernelGGL(theta_to_svd, dim3(grid_for(n)), dim3(256), 0, s, th, X, chil, chir, mode);
+  QCA_HIP(hipGetLastError());
+}
+
+void launch_gather_isometry(const c128* X, const int* perm, const double* sigma, c128* A, int rows,
+                            int chi_bond, int k, int mode, hipStream_t s) {
+  const long n = (long)rows * k;
+  if (n == 0) return;
+  hipLaunchKernelGGL(gather_isometry, dim3(grid_for(n)), dim3(256), 0, s, X, perm, sigma, A, rows,
+                     chi_bond, k, mode);
+  QCA_HIP(hipGetLastError());
+}
+
+int lanczos_nblk(long n) { return (int)std::max<long>(1, (n + LZ_ELEMS - 1) / LZ_ELEMS); }
+
+const int* lanczos_done_flag(const LanczosWork& lw) { return &lw.ctrl->done; }
+
+void lanczos_begin(const c128* x, long n, LanczosWork& lw, hipStream_t s) {
+  const int nb = lanczos_nblk(n);
+  QCA_CHECK(nb <= lw.max_blocks, "Lanczos vector longer than the workspace");
+  hipLaunchKernelGGL(lz_norm_partials, dim3(nb), dim3(LZ_BLOCK), 0, s, x, n, lw.part_norm);
+  hipLaunchKernelGGL(lz_init_finalize, dim3(1), dim3(64), 0, s, lw.ctrl, lw.part_norm, nb);
+  hipLaunchKernelGGL(lz_scale_first, dim3(grid_for(n)), dim3(256), 0, s, x, lw.V, n,
+                     (const LanczosCtrl*)lw.ctrl);
+  QCA_HIP(hipGetLastError());
+}
+
+template <typename TW>
+void lanczos_iteration_tail(const cplx<TW>* w, long n, int j, double tau, double tol,
+                            LanczosWork& lw, hipStream_t s) {
+  QCA_CHECK(j + 1 < lw.max_m && lw.max_m <= LZ_MAXK, "Krylov dimension limit");
+  const int nb = lanczos_nblk(n);
+  const LanczosCtrl* cc = lw.ctrl;
+  hipLaunchKernelGGL(lz_project<TW>, dim3(nb), dim3(LZ_BLOCK), 0, s, cc, w, lw.V, n, j, lw.part);
+  hipLaunchKernelGGL(lz_reduce_coeffs, dim3(1), dim3(LZ_MAXK), 0, s, cc, lw.part, nb, j, lw.coef1);
+  hipLaunchKernelGGL(lz_subtract<TW>, dim3(nb), dim3(LZ_BLOCK), 0, s, cc, w, lw.w64, lw.V, n, j,
+                     lw.coef1, lw.part, 1);
+  hipLaunchKernelGGL(lz_reduce_coeffs, dim3(1), dim3(LZ_MAXK), 0, s, cc, lw.part, nb, j, lw.coef2);
+  hipLaunchKernelGGL(lz_subtract<double>, dim3(nb), dim3(LZ_BLOCK), 0, s, cc, lw.w64, lw.w64, lw.V,
+                     n, j, lw.coef2, lw.part, 2);
+  hipLaunchKernelGGL(lz_step, dim3(1), dim3(64), 0, s, lw.ctrl, lw.part, nb, lw.coef1, lw.coef2, j,
+                     tau, tol, lw.max_m);
+  hipLaunchKernelGGL(lz_scale_next, dim3(grid_for(n)), dim3(256), 0, s, cc, lw.w64,
+                     lw.V + (long)(j + 1) * n, n, j);
+  QCA_HIP(hipGetLastError());
+}
+template void lanczos_iteration_tail<double>(const c128*, long, int, double, double, LanczosWork&,
+                                             hipStream_t);
+template void lanczos_iteration_tail<float>(const c64*, long, int, double, double, LanczosWork&,
+                                            hipStream_t);
+
+void lanczos_finish(long n, c128* y, LanczosWork& lw, hipStream_t s) {
+  hipLaunchKernelGGL(lz_combine, dim3(grid_for(n)), dim3(256), 0, s, (const LanczosCtrl*)lw.ctrl,
+                     lw.V, n, y);
+  QCA_HIP(hipGetLastError());
+}
+
+void launch_rho(const c128* A, const c128* T, long nxb, c128* rho, hipStream_t s) {
+  hipLaunchKernelGGL(rho_kernel, dim3(1), dim3(256), 0, s, A, T, nxb, rho);
+  QCA_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_exact_apply(const cplx<T>* psi, cplx<T>* out, int ncells, int distance, int lo, int hi,
+                        hipStream_t s) {
+  const long dim = 1L << ncells;
+  hipLaunchKernelGGL(exact_apply_kernel<T>, dim3(grid_for(dim, 256, 65536)), dim3(256), 0, s, psi,
+                     out, ncells, distance, lo, hi);
+  QCA_HIP(hipGetLastError());
+}
+template void launch_exact_apply<double>(const c128*, c128*, int, int, int, int, hipStream_t);
+
+int exact_rho_blocks(int ncells) {
+  const long dim = 1L << ncells;
+  return (int)std::max<long>(1, std::min<long>((dim + 255) / 256, 512));
+}
+
+void launch_exact_rho(const c128* psi, int ncells, double* part, c128* rho, hipStream_t s) {
+  const int nbx = exact_rho_blocks(ncells);
+  hipLaunchKernelGGL(exact_rho_partials, dim3(nbx, ncells), dim3(256), 0, s, psi, ncells, part);
+  hipLaunchKernelGGL(exact_rho_finalize, dim3((ncells + 63) / 64), dim3(64), 0, s, part, nbx, ncells, rho);
+  QCA_HIP(hipGetLastError());
+}
+template void launch_exact_apply<float>(const c64*, c64*, int, int, int, int, hipStream_t);
+
+}  // namespace qca

@@ -1,0 +1,108 @@
+// Device-side data structures and kernel launchers of the TDVP path.
+#pragma once
+
+#include "qca_common.hpp"
+
+namespace qca {
+
+// Maximum Krylov dimension (one wave holds the tridiagonal matrix).
+constexpr int LZ_MAXK = 64;
+
+// Lanczos control block, device resident.  Written by the reduction kernels,
+// read back by the host only once per exponential.
+struct LanczosCtrl {
+  double beta0;
+  double err;
+  int done;
+  int m;
+  int zero;
+  int hit_limit;
+  double alpha[LZ_MAXK];
+  double beta[LZ_MAXK];
+  c128 z[LZ_MAXK];
+};
+
+// Workspace of one Lanczos exponential (vectors of length <= max_n).
+struct LanczosWork {
+  LanczosCtrl* ctrl = nullptr;
+  c128* V = nullptr;      // [LZ_MAXK][max_n] Krylov basis
+  c128* w64 = nullptr;    // [max_n] orthogonalised residual
+  c128* part = nullptr;   // [max_blocks][LZ_MAXK] partial projections
+  double* part_norm = nullptr;
+  c128* coef1 = nullptr;  // [LZ_MAXK]
+  c128* coef2 = nullptr;
+  long max_n = 0;
+  int max_blocks = 0;
+  int max_m = LZ_MAXK;    // Krylov dimension limit of this workspace (<= LZ_MAXK)
+};
+
+// MPO tensor in compressed sparse rows over output channels: a channel is
+// (physical, MPO bond) and the value of each stored entry is a W element.
+template <typename T>
+struct WSparseDev {
+  int* rowptr = nullptr;
+  int* col = nullptr;
+  cplx<T>* val = nullptr;
+  int nout = 0, nin = 0, nnz = 0;
+};
+
+template <typename T>
+void launch_wmix(const cplx<T>* Tin, long in_stride, long nxc, int nin, cplx<T>* U, int nq,
+                 int Dout, const WSparseDev<T>& w, hipStream_t s, const int* skip = nullptr);
+
+void launch_convert_c128_c64(const c128* in, c64* out, long n, hipStream_t s,
+                             const int* skip = nullptr);
+void launch_convert_c64_c128(const c64* in, c128* out, long n, hipStream_t s,
+                             const int* skip = nullptr);
+void launch_to_colmajor(const c128* in, c128* out, int rows, int cols, hipStream_t s);
+void launch_site_to_rightqr(const c128* A, c128* X, int chil, int chir, hipStream_t s);
+void launch_qr_gather_left(const c128* Q, const c128* X, c128* Anew, c128* C, int rows, int cols,
+                           int qcols, int target, hipStream_t s);
+void launch_qr_gather_right(const c128* Q, const c128* X, c128* Anew, c128* C, int chir, int cols,
+                            int qcols, int target, hipStream_t s);
+void launch_theta_to_svd(const c128* th, c128* X, int chil, int chir, int mode, hipStream_t s);
+void launch_gather_isometry(const c128* X, const int* perm, const double* sigma, c128* A, int rows,
+                            int chi_bond, int k, int mode, hipStream_t s);
+
+int lanczos_nblk(long n);
+const int* lanczos_done_flag(const LanczosWork& lw);
+void lanczos_begin(const c128* x, long n, LanczosWork& lw, hipStream_t s);
+template <typename TW>
+void lanczos_iteration_tail(const cplx<TW>* w, long n, int j, double tau, double tol,
+                            LanczosWork& lw, hipStream_t s);
+void lanczos_finish(long n, c128* y, LanczosWork& lw, hipStream_t s);
+
+void launch_rho(const c128* A, const c128* T, long nxb, c128* rho, hipStream_t s);
+
+int exact_rho_blocks(int ncells);
+void launch_exact_rho(const c128* psi, int ncells, double* part, c128* rho, hipStream_t s);
+
+template <typename T>
+void launch_exact_apply(const cplx<T>* psi, cplx<T>* out, int ncells, int distance, int lo, int hi,
+                        hipStream_t s);
+
+// ---------------------------------------------------------------- SVD / QR
+struct SvdResult {
+  int k;              // truncation rank
+  int nsweeps;        // Jacobi sweeps used
+  int converged;
+  int pad;
+  double norm_kept;   // ||s[:k]||
+  double tail;        // ||s[k:]||
+};
+
+// One-sided Jacobi SVD of the column-major matrix X (rows x cols, FP64),
+// in place.  On exit the columns are orthogonal; sigma (sorted descending),
+// perm (column index of each sorted value) and the truncation result are
+// written to device memory.
+void launch_jacobi_svd(c128* X, int rows, int cols, double* sigma, int* perm, SvdResult* res,
+                       int chi_max, double eps, int rank_cap, hipStream_t s);
+
+// Householder QR with LAPACK's zgeqr2/zung2r conventions (real beta, beta =
+// -sign(Re alpha)*norm) of the column-major matrix M (rows x cols), in place:
+// R on and above the diagonal, reflectors below it, tau[min(rows, cols)].
+// Q's first qcols columns are written column-major to Q (skipped when null).
+void launch_householder_qr(c128* M, int rows, int cols, c128* tau, c128* Q, int qcols,
+                           hipStream_t s);
+
+}  // namespace qca

@@ -1,0 +1,63 @@
+// Shared definitions for the gfx950 TDVP kernels and the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+namespace qca {
+
+// Complex numbers are stored interleaved (re, im), the layout of
+// std::complex<T> and of torch.complex64 / complex128.
+template <typename T>
+struct cplx {
+  T re, im;
+};
+using c64 = cplx<float>;
+using c128 = cplx<double>;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define QCA_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      throw ::qca::Error(-3, std::string(#call " failed: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+#define QCA_CHECK(cond, msg)                                                       \
+  do {                                                                             \
+    if (!(cond)) throw ::qca::Error(-2, std::string("invalid argument: ") + (msg)); \
+  } while (0)
+
+inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------- GEMM descriptor
+// C[b](m,n) = alpha * sum_k opA(A)[b](m,k) * opB(B)[b](k,n) + beta * C[b](m,n)
+// with every index split into at most two levels so that contractions over
+// composite indices (e.g. (MPO bond, bond) or (physical, bond)) map onto one
+// GEMM without a transpose copy.  k = k1*K2 + k2.  batch = b1*NB2 + b2.
+struct GemmDesc {
+  int M, N, K, K2;  // K2 divides K
+  int NB1, NB2;     // batch = NB1 * NB2
+  long a_m, a_k1, a_k2, a_b1, a_b2;
+  long b_k1, b_k2, b_n, b_b1, b_b2;
+  long c_m, c_n, c_b1, c_b2;
+  int conj_a, conj_b;
+  double alpha_re, alpha_im;
+  int accumulate;  // beta = 1 when set, else 0
+  const int* skip; // device flag: launch is a no-op when *skip != 0 (may be null)
+};
+
+void gemm_c128(const GemmDesc& d, const c128* A, const c128* B, c128* C, hipStream_t s,
+               c128* workspace, long workspace_elems);
+void gemm_c64(const GemmDesc& d, const c64* A, const c64* B, c64* C, hipStream_t s,
+              c64* workspace, long workspace_elems);
+
+}  // namespace qca

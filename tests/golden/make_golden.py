@@ -49,8 +49,22 @@ def _meta():
                 openblas_threads=os.environ.get("OPENBLAS_NUM_THREADS", ""))
 
 
+def padded_tensors(plist, chi, scale, seed):
+    """Bond-padded product state (complex Gaussian filler of size ``scale``)."""
+    n = len(plist)
+    rng = np.random.default_rng(seed)
+    dims = [min(2 ** i, 2 ** (n - i), chi) for i in range(n + 1)]
+    out = []
+    for i, p in enumerate(plist):
+        shape = (2, dims[i], dims[i + 1])
+        A = scale * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / np.sqrt(2)
+        A[:, 0, 0] = ((1.0 - p) ** 0.5, p ** 0.5)
+        out.append(A)
+    return out
+
+
 def run_reference(algorithm, ncells, distance, lo, hi, plist, dt, steps, chi, eps,
-                  plot_interval):
+                  plot_interval, tensors=None):
     """Replay quantum_game.py:82-119 for one initial state.
 
     ``lo, hi`` is the half-open CLI activation interval (parser.py:182).
@@ -68,7 +82,8 @@ def run_reference(algorithm, ncells, distance, lo, hi, plist, dt, steps, chi, ep
     ARGS.plot_step_interval = plot_interval
     plot_steps = steps // plot_interval + (1 if steps % plot_interval else 0)
     H = MPO.hamiltonian_from_rules(rules)
-    psi0 = MPS.from_density_distribution(plist=list(plist))
+    psi0 = (MPS.from_density_distribution(plist=list(plist)) if tensors is None
+            else MPS.from_tensors([a.copy() for a in tensors]))
     pop = np.zeros((plot_steps, ncells))
     dpop = np.zeros((plot_steps, ncells))
     sse = np.zeros((plot_steps, ncells))
@@ -261,7 +276,31 @@ def g_units():
     _save("units", **out)
 
 
-ALL = dict(mpo=g_mpo, exact=g_exact, fullchi=g_fullchi, units=g_units, short=g_short,
+def g_padded():
+    """1TDVP and 2TDVP from bond-padded product states (well-conditioned
+    padding subspace), measured every step and every 5 steps."""
+    out = {}
+    cases = [
+        ("1tdvp", 10, 1, 1, 2, 0.1, 6, 8, 1e-10, 1, 11),
+        ("1tdvp", 9, 2, 2, 4, 0.05, 10, 8, 1e-10, 5, 12),
+        ("2tdvp", 10, 1, 1, 2, 0.1, 6, 8, 1e-10, 1, 13),
+        ("2tdvp", 11, 2, 2, 5, 0.1, 10, 8, 1e-10, 5, 14),
+    ]
+    for i, (alg, n, d, lo, hi, dt, steps, chi, eps, itv, seed) in enumerate(cases):
+        rng = np.random.default_rng(seed)
+        pl = [1.0 if x < 0.3 else 0.0 for x in rng.random(n)]
+        As = padded_tensors(pl, chi, 1e-3, seed)
+        r = run_reference(alg, n, d, lo, hi, pl, dt, steps, chi, eps, itv, tensors=As)
+        out[f"c{i}_alg"] = np.array(alg)
+        out[f"c{i}_params"] = np.array([n, d, lo, hi, dt, steps, chi, eps, itv], dtype=float)
+        for j, A in enumerate(As):
+            out[f"c{i}_A{j}"] = A
+        for k in ("population", "d_population", "single_site_entropy", "bond_dims"):
+            out[f"c{i}_{k}"] = r[k]
+    _save("padded_runs", **out)
+
+
+ALL = dict(padded=g_padded, mpo=g_mpo, exact=g_exact, fullchi=g_fullchi, units=g_units, short=g_short,
            config1=g_config1)
 
 if __name__ == "__main__":

@@ -1,0 +1,81 @@
+"""Host-side modules of the drop-in (no GPU): MPO construction, parameters,
+MPS conversions, observables post-processing, classical overlay."""
+import numpy as np
+import pytest
+
+from conftest import crand, golden
+
+from parameters import Parser, Rules
+from tensor_networks import MPO, MPS
+from algorithms.algorithm import Algorithm, entropy_bits, write_observables
+from oracle import automaton as AU, mps as OM
+
+
+def test_mpo_builder_matches_reference_golden():
+    g = golden("mpo_tensors")
+    for k in g.files:
+        if not k.endswith("_bulk"):
+            continue
+        d, a, b = k.split("_")[:3]
+        d, a, b = int(d[1:]), int(a), int(b)
+        Ws = MPO.hamiltonian_from_rules(Rules(6, range(a, b), d, False)).W
+        for tag, W in (("first", Ws[0]), ("bulk", Ws[1]), ("last", Ws[-1])):
+            assert np.array_equal(W, g[f"d{d}_{a}_{b}_{tag}"]), (k, tag)
+
+
+def test_mpo_matrix_is_closed_form():
+    H = MPO.hamiltonian_from_rules(Rules(7, range(2, 5), 2, False)).as_matrix()
+    assert np.abs(H - AU.dense_h(7, 2, range(2, 5))).max() == 0.0
+
+
+def test_parser_defaults_and_cadence():
+    p = Parser([])
+    assert p.rules.ncells == 9 and p.rules.distance == 1
+    assert p.rules.activation_interval == range(1, 2)
+    assert p.algorithm == "exact" and p.max_bond_dim == 32 and p.svd_epsilon == 5e-5
+    assert p.plot_step_interval == 200 and p.plot_steps == 50
+    q = Parser(["--num-cells", "15", "--algorithm", "2tdvp", "--step-size", "0.1", "--num-steps", "100",
+                "--max-bond-dim", "16", "--svd-epsilon", "1e-10", "--plotting-frequency", "10",
+                "--precision", "mixed"])
+    assert q.plot_step_interval == 1 and q.plot_steps == 100 and q.precision == "mixed"
+    with pytest.raises(SystemExit):
+        Parser(["--taylor-steps", "0"])
+
+
+def test_mps_vector_roundtrip_and_file(tmp_path):
+    rng = np.random.default_rng(0)
+    n = 6
+    psi = crand(rng, 1 << n)
+    psi /= np.linalg.norm(psi)
+    m = MPS.from_vector(psi)
+    assert np.abs(m.as_vector() - psi).max() < 1e-13
+    assert m.bond_dims() == [1, 2, 4, 8, 4, 2, 1]
+    path = tmp_path / "state.npz"
+    m.write_to_file(str(path))
+    m2 = MPS.from_file(str(path))
+    assert all(np.array_equal(a, b) for a, b in zip(m.A, m2.A))
+    p = MPS.from_density_distribution([0.0, 1.0, 0.3])
+    assert np.allclose(np.abs(p.as_vector()) ** 2, np.abs(OM.as_vector(OM.product_state([0.0, 1.0, 0.3]))) ** 2)
+
+
+def test_observables_post_processing():
+    rho = np.zeros((3, 2, 2), dtype=complex)
+    rho[0] = [[1, 0], [0, 0]]
+    rho[1] = [[0.5, 0], [0, 0.5]]
+    rho[2] = [[0.3, 0.2j], [-0.2j, 0.7]]
+    pop, dp, sse, bd = np.zeros(3), np.zeros(3), np.zeros(3), np.zeros(4)
+    write_observables(rho, [1, 2, 2, 1], pop, dp, sse, bd)
+    assert np.allclose(pop, [0, 0.5, 0.7]) and np.allclose(dp, [0, 0, 1])
+    assert abs(sse[0]) < 1e-15 and abs(sse[1] - 1.0) < 1e-15
+    w = np.linalg.eigvalsh(rho[2])
+    assert abs(sse[2] + (w * np.log2(w)).sum()) < 1e-14
+    assert list(bd) == [1, 2, 2, 1]
+    assert entropy_bits(np.array([[0.0, 0.0], [0.0, 1.0]])) == 0.0
+
+
+def test_classical_overlay():
+    rules = Rules(7, range(1, 2), 1, False)
+    first = np.array([0, 0, 0, 1, 0, 0, 0], dtype=float)
+    h = Algorithm.classical_evolution(first, rules, 3)
+    assert list(h[1]) == [0, 0, 1, 1, 1, 0, 0]
+    assert list(h[2]) == [0, 1, 0, 1, 0, 1, 0]

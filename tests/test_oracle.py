@@ -1,0 +1,276 @@
+"""Pins of the oracle (CPU restatement) against the reference's golden vectors,
+closed forms and invariants.  Each check is chosen so that a plausible mistake
+(dropped term, wrong sign or index, transposed operand) fails it; the last
+tests plant such mistakes in the oracle and assert that the pins catch them.
+"""
+import numpy as np
+import pytest
+import scipy.linalg
+
+from conftest import crand, golden
+
+from oracle import automaton as AU, exact as OE, local as LOC, mps as OM, tdvp as OT
+
+
+# ---------------------------------------------------------------- automaton / H
+
+def mpo_keys():
+    g = golden("mpo_tensors")
+    out = []
+    for k in g.files:
+        if k.endswith("_bulk"):
+            d, a, b = k.split("_")[:3]
+            out.append((int(d[1:]), int(a), int(b)))
+    return out
+
+
+@pytest.mark.parametrize("d,a,b", mpo_keys())
+def test_mpo_matches_reference(d, a, b):
+    g = golden("mpo_tensors")
+    Ws = AU.mpo_tensors(6, d, range(a, b))
+    for tag, W in (("first", Ws[0]), ("bulk", Ws[1]), ("last", Ws[-1])):
+        assert np.array_equal(W, g[f"d{d}_{a}_{b}_{tag}"])
+
+
+def test_bond_dimensions_of_appendix_c():
+    expect = {(1, range(1, 2)): 6, (1, range(1, 3)): 6, (1, range(2, 3)): 4, (2, range(1, 2)): 10,
+              (2, range(2, 3)): 12, (2, range(2, 4)): 13, (2, range(2, 5)): 13, (2, range(1, 5)): 14,
+              (2, range(4, 5)): 6}
+    for (d, iv), D in expect.items():
+        assert AU.bulk_tensor(d, iv).shape[2] == D
+
+
+@pytest.mark.parametrize("n,d,iv", [(5, 1, range(1, 2)), (7, 2, range(2, 5)), (8, 2, range(1, 3)),
+                                    (9, 1, range(0, 2)), (6, 2, range(4, 5))])
+def test_closed_form_equals_mpo(n, d, iv):
+    """The 2^N matrix of the MPO equals the bit-flip closed form."""
+    Hm = AU.mpo_dense(AU.mpo_tensors(n, d, iv))
+    Hc = AU.dense_h(n, d, iv)
+    assert np.abs(Hm - Hc).max() == 0.0
+    assert np.abs(Hc - Hc.T).max() == 0.0
+
+
+def test_closed_form_brute_force():
+    """Direct count of alive neighbours for every basis state, N=7, d=2."""
+    n, d, iv = 7, 2, range(2, 4)
+    H = AU.dense_h(n, d, iv).real
+    for x in range(1 << n):
+        bits = [(x >> (n - 1 - j)) & 1 for j in range(n)]
+        for i in range(n):
+            cnt = sum(bits[j] for j in range(max(0, i - d), min(n, i + d + 1)) if j != i)
+            y = x ^ (1 << (n - 1 - i))
+            assert H[y, x] == (1.0 if cnt in iv else 0.0)
+
+
+# ---------------------------------------------------------------- exact
+
+def test_exact_matches_reference():
+    g = golden("exact_small")
+    for c in range(5):
+        n, d, lo, hi, dt, steps, itv = g[f"c{c}_params"]
+        n, d, lo, hi, steps, itv = int(n), int(d), int(lo), int(hi), int(steps), int(itv)
+        pop = OE.run_exact(OM.product_state(g[f"c{c}_plist"]), n, d, range(lo, hi), dt, steps, itv)
+        assert np.abs(pop - g[f"c{c}_population"]).max() < 1e-12
+        pk = OE.run_exact(OM.product_state(g[f"c{c}_plist"]), n, d, range(lo, hi), dt, steps, itv,
+                          mode="krylov")
+        assert np.abs(pk - g[f"c{c}_population"]).max() < 1e-12
+
+
+def test_exact_analytic_three_level():
+    """N=2, d=1, [1,1]: H is the path-graph adjacency on |01>,|11>,|10>."""
+    n, d, iv = 2, 1, range(1, 2)
+    for t in (0.1, 0.37, 1.3):
+        ex = OE.ExactOracle(OM.product_state([0.0, 1.0]), n, d, iv, t)
+        ex.do_time_step()
+        th = np.pi / 2 * t
+        c, s = np.cos(np.sqrt(2) * th), np.sin(np.sqrt(2) * th)
+        n1 = ((1 + c) / 2) ** 2 + s ** 2 / 2
+        n0 = s ** 2 / 2 + ((1 - c) / 2) ** 2
+        assert np.allclose(ex.populations(), [n0, n1], atol=1e-14)
+
+
+def test_exact_reflection_symmetry():
+    n, d, iv = 11, 2, range(2, 5)
+    pl = [0.0] * n
+    pl[4] = pl[5] = pl[6] = 1.0
+    pop = OE.run_exact(OM.product_state(pl), n, d, iv, 0.1, 15, 1, mode="krylov")
+    assert np.abs(pop - pop[:, ::-1]).max() < 1e-12
+
+
+# ---------------------------------------------------------------- local operations
+
+def test_env_updates_match_reference():
+    g = golden("units")
+    W = g["W"]
+    assert np.abs(LOC.env_left(g["env_Lprev"], g["env_A"], W) - g["env_Lnew"]).max() < 1e-12
+    assert np.abs(LOC.env_right(g["env_Rprev"], g["env_A"], W) - g["env_Rnew"]).max() < 1e-12
+
+
+def test_site_and_bond_evolution_match_reference():
+    g = golden("units")
+    W = g["W"]
+    for key, delta in (("site_fwd", 0.05), ("site_bwd", -0.05)):
+        Hm = LOC.dense_heff(g["site_L"], g["site_R"], W)
+        x = g["site_A"]
+        got = LOC.expm_dense(Hm, x.reshape(-1), delta).reshape(x.shape)
+        assert np.abs(got - g[key]).max() < 1e-13
+        gk, _ = LOC.expm_krylov(lambda v: LOC.heff_apply(v.reshape(x.shape), g["site_L"], g["site_R"], W).reshape(-1),
+                                x.reshape(-1), delta)
+        assert np.abs(gk.reshape(x.shape) - g[key]).max() < 1e-13
+    C = g["bond_C"]
+    got = LOC.expm_dense(LOC.dense_keff(g["bond_L"], g["bond_R"]), C.reshape(-1), float(g["bond_step"]))
+    assert np.abs(got.reshape(C.shape) - g["bond_out"]).max() < 1e-13
+
+
+@pytest.mark.parametrize("tag", ["a", "b", "c"])
+def test_split_matches_reference(tag):
+    g = golden("units")
+    W = g["W"]
+    Al, Ar, L, R = g["two_Al"], g["two_Ar"], g["two_L"], g["two_R"]
+    chi, eps = g[f"two_{tag}_chi_eps"]
+    theta = LOC.merge_theta(Al, Ar)
+    W2 = LOC.merge_w(W, W)
+    theta = LOC.expm_dense(LOC.dense_heff(L, R, W2), theta.reshape(-1), 0.05).reshape(theta.shape)
+    u, s, v = LOC.split_theta(theta, Al.shape[1], Ar.shape[2], int(chi), eps)
+    ru, rs, rv = g[f"two_{tag}_u"], g[f"two_{tag}_s"], g[f"two_{tag}_v"]
+    assert u.shape == ru.shape and v.shape == rv.shape
+    assert np.abs(s - rs).max() < 1e-13
+    # gauge-independent truncated two-site tensor
+    got = np.einsum("pak,k,qkc->pqac", u, s, v)
+    ref = np.einsum("pak,k,qkc->pqac", ru, rs, rv)
+    assert np.abs(got - ref).max() < 1e-12
+
+
+@pytest.mark.parametrize("n", [6, 40, 200])
+def test_krylov_vs_scipy_expm(n):
+    rng = np.random.default_rng(n)
+    Hm = crand(rng, n, n)
+    Hm = Hm + Hm.conj().T
+    v = crand(rng, n)
+    for delta in (0.05, -0.3):
+        ref = scipy.linalg.expm(-1j * np.pi / 2 * delta * Hm) @ v
+        got, _ = LOC.expm_krylov(lambda x: Hm @ x, v, delta)
+        dense = np.tensordot(v, scipy.linalg.expm(-1j * np.pi / 2 * delta * Hm.T), (0, 0))
+        assert np.abs(got - ref).max() < 1e-12
+        assert np.abs(LOC.expm_dense(Hm.T, v, delta) - ref).max() < 1e-12
+        assert np.abs(dense - ref).max() < 1e-12
+
+
+def test_truncation_rank_rule():
+    s = np.array([0.9, 0.4, 1e-6, 1e-9, 1e-11, 1e-12])
+    assert LOC.truncation_rank(s, 10, 1e-10) == 4
+    assert LOC.truncation_rank(s, 3, 1e-10) == 3
+    assert LOC.truncation_rank(s, 10, 1e-5) == 2
+    assert LOC.truncation_rank(s, 10, 0.0) == 6
+
+
+# ---------------------------------------------------------------- TDVP
+
+def short_case(c):
+    g = golden("short_runs")
+    alg = str(g[f"c{c}_alg"])
+    n, d, lo, hi, dt, steps, chi, eps, itv = g[f"c{c}_params"]
+    return g, alg, int(n), int(d), range(int(lo), int(hi)), dt, int(steps), int(chi), eps, int(itv)
+
+
+@pytest.mark.parametrize("c", [0, 1, 2, 3, 4, 5])
+def test_first_measured_rows_match_reference(c):
+    """Both gauge modes perform the reference's first step identically (from a
+    product state the re-canonicalisation only flips signs)."""
+    g, alg, n, d, iv, dt, steps, chi, eps, itv = short_case(c)
+    rows = 2 if itv == 1 else 1
+    for gauge in ("reference", "consistent"):
+        o = OT.TDVPOracle(OM.product_state(g[f"c{c}_plist"]), AU.mpo_tensors(n, d, iv), alg, dt, chi, eps,
+                          gauge=gauge)
+        r = OT.run_trajectory(o, steps, itv)
+        if alg == "2tdvp":
+            assert np.abs(r["population"][:rows] - g[f"c{c}_population"][:rows]).max() < 1e-12
+        assert np.array_equal(r["bond_dims"][:rows], g[f"c{c}_bond_dims"][:rows])
+
+
+def test_reference_mode_replays_whole_trajectory():
+    """Where the reference's gauge choices are numerically well determined the
+    reference-mode replay reproduces its whole trajectory (stale environments
+    included): 2TDVP short run c3 and 1TDVP full-chi measured every 10 steps."""
+    g, alg, n, d, iv, dt, steps, chi, eps, itv = short_case(3)
+    o = OT.TDVPOracle(OM.product_state(g["c3_plist"]), AU.mpo_tensors(n, d, iv), alg, dt, chi, eps,
+                      gauge="reference")
+    r = OT.run_trajectory(o, steps, itv)
+    assert np.abs(r["population"] - g["c3_population"]).max() < 1e-12
+    assert np.array_equal(r["bond_dims"], g["c3_bond_dims"])
+    f = golden("fullchi_n8")
+    pl = [0.0] * 8
+    pl[4] = 1.0
+    o = OT.TDVPOracle(OM.product_state(pl), AU.mpo_tensors(8, 1, range(1, 2)), "1tdvp", 0.1, 16, 1e-12,
+                      gauge="reference")
+    r = OT.run_trajectory(o, 20, 10)
+    assert np.abs(r["population"] - f["1tdvp_itv10_population"]).max() < 1e-12
+
+
+def test_reference_trajectory_is_not_exact():
+    """The reference's own TDVP at full bond dimension deviates from its Exact
+    (stale right environments after re-canonicalisation and measurement)."""
+    f = golden("fullchi_n8")
+    for alg in ("2tdvp", "1tdvp"):
+        assert np.abs(f[f"{alg}_itv1_population"] - f["exact_itv1_population"]).max() > 1e-4
+
+
+def full_rank_state(n, seed):
+    rng = np.random.default_rng(seed)
+    dims = [min(2 ** i, 2 ** (n - i)) for i in range(n + 1)]
+    As = [crand(rng, 2, dims[i], dims[i + 1]) for i in range(n)]
+    As[0] = As[0] / np.linalg.norm(OM.as_vector(As))
+    return As
+
+
+@pytest.mark.parametrize("alg", ["2tdvp", "1tdvp"])
+@pytest.mark.parametrize("expm", ["dense", "lanczos"])
+def test_consistent_full_rank_is_exact(alg, expm):
+    """Projector-splitting TDVP at full bond dimension on a full-rank state is
+    exact (Lubich, Oseledets, Vandereycken 2015): the consistent mode must
+    reproduce the exact 2^N evolution."""
+    n, d, iv, dt = 7, 1, range(1, 2), 0.1
+    As = full_rank_state(n, 1)
+    o = OT.TDVPOracle([a.copy() for a in As], AU.mpo_tensors(n, d, iv), alg, dt, 8, 1e-14,
+                      gauge="consistent", expm=expm)
+    ex = OE.ExactOracle([a.copy() for a in As], n, d, iv, dt)
+    for _ in range(4):
+        assert np.abs(o.measure()[0] - ex.populations()).max() < 1e-12
+        o.do_time_step()
+        ex.do_time_step()
+
+
+def test_consistent_measure_equals_reference_measure():
+    n = 8
+    As = full_rank_state(n, 2)
+    o = OT.TDVPOracle([a.copy() for a in As], AU.mpo_tensors(n, 1, range(1, 2)), "2tdvp", 0.1, 16, 1e-12,
+                      gauge="consistent")
+    o.do_time_step()
+    pc, _, sc, bc = OM.measure_consistent([a.copy() for a in o.A])
+    pr, _, sr, br = OM.measure_reference([a.copy() for a in o.A])
+    assert np.abs(pc - pr).max() < 1e-13
+    assert np.abs(sc - sr).max() < 1e-9
+    assert np.array_equal(bc, br)
+
+
+# ---------------------------------------------------------------- planted mistakes
+
+def test_planted_mistakes_are_caught(monkeypatch):
+    """Mistakes planted in the oracle on the CPU must fail the pins above."""
+    g = golden("units")
+    W = g["W"]
+    # (1) transposed MPO physical indices in the left environment update
+    bad = np.einsum("pqwv,awx,pab,qxy->bvy", W.transpose(1, 0, 2, 3) * np.array([1, 2]).reshape(2, 1, 1, 1),
+                    g["env_Lprev"], g["env_A"], g["env_A"].conj())
+    assert np.abs(bad - g["env_Lnew"]).max() > 1e-6
+    # (2) wrong sign of the propagator exponent
+    Hm = LOC.dense_heff(g["site_L"], g["site_R"], W)
+    x = g["site_A"]
+    wrong = LOC.expm_dense(Hm, x.reshape(-1), -0.05).reshape(x.shape)
+    assert np.abs(wrong - g["site_fwd"]).max() > 1e-6
+    # (3) dropping the boundary dead cells changes the MPO
+    Wb = AU.bulk_tensor(2, range(2, 5))
+    assert Wb.shape != golden("mpo_tensors")["d2_2_5_first"].shape
+    # (4) a wrong neighbour window in the closed form breaks MPO equality
+    monkeypatch.setattr(AU, "neighbour_masks", lambda n, d: [((1 << n) - 1) ^ (1 << (n - 1 - i)) for i in range(n)])
+    assert np.abs(AU.dense_h(6, 1, range(1, 2)) - AU.mpo_dense(AU.mpo_tensors(6, 1, range(1, 2)))).max() > 0
