@@ -388,7 +388,9 @@ int qca_svd_split(const void* theta, void* Al, void* Ar, int chil, int chir, int
     int* perm = sbuf<int>("svd_perm", cols);
     qca::SvdResult* res = sbuf<qca::SvdResult>("svd_res", 1);
     qca::launch_theta_to_svd(th, X, chil, chir, right_sweep ? 0 : 1, s);
-    qca::launch_jacobi_svd(X, rows, cols, sigma, perm, res, chi_max, eps, rank_cap, s);
+    qca::JacobiWork jw;
+    jw.sync = sbuf<unsigned char>("svd_sync", 64);
+    qca::launch_jacobi_svd(X, rows, cols, sigma, perm, res, chi_max, eps, rank_cap, s, jw);
     qca::SvdResult hres;
     QCA_HIP(hipMemcpyAsync(&hres, res, sizeof(hres), hipMemcpyDeviceToHost, s));
     QCA_HIP(hipStreamSynchronize(s));

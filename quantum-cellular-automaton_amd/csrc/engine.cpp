@@ -152,6 +152,7 @@ Engine::Engine(int device, const EngineParams& p, const std::vector<const c128*>
   sigma_ = dmalloc<double>(2 * cmax);
   perm_ = dmalloc<int>(2 * cmax);
   svdres_ = dmalloc<SvdResult>(1);
+  jw_.sync = dmalloc<unsigned char>(64);
   QCA_HIP(hipHostMalloc((void**)&svdres_host_, sizeof(SvdResult)));
   QCA_HIP(hipHostMalloc((void**)&ctrl_host_, sizeof(LanczosCtrl)));
   lw_.max_n = nmax;
@@ -327,7 +328,7 @@ int Engine::split_theta(int s, bool right_sweep, int chil, int chir) {
   const int cols = right_sweep ? 2 * chir : 2 * chil;
   const int rank_cap = std::min(2 * chil, 2 * chir);
   launch_theta_to_svd(theta_, svdX_, chil, chir, right_sweep ? 0 : 1, s_);
-  launch_jacobi_svd(svdX_, rows, cols, sigma_, perm_, svdres_, p_.chi_max, p_.eps, rank_cap, s_);
+  launch_jacobi_svd(svdX_, rows, cols, sigma_, perm_, svdres_, p_.chi_max, p_.eps, rank_cap, s_, jw_);
   QCA_HIP(hipMemcpyAsync(svdres_host_, svdres_, sizeof(SvdResult), hipMemcpyDeviceToHost, s_));
   sync();
   const int k = svdres_host_->k;

@@ -106,6 +106,7 @@ class Engine {
   double* sigma_ = nullptr;
   int* perm_ = nullptr;
   SvdResult* svdres_ = nullptr;
+  JacobiWork jw_;
   SvdResult* svdres_host_ = nullptr;
   LanczosCtrl* ctrl_host_ = nullptr;
   LanczosWork lw_;

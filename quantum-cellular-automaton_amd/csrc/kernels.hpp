@@ -91,12 +91,17 @@ struct SvdResult {
   double tail;        // ||s[k:]||
 };
 
-// One-sided Jacobi SVD of the column-major matrix X (rows x cols, FP64),
-// in place.  On exit the columns are orthogonal; sigma (sorted descending),
-// perm (column index of each sorted value) and the truncation result are
-// written to device memory.
+// Device scratch of the Jacobi SVD: 64 bytes of barrier / counter words.
+struct JacobiWork {
+  void* sync = nullptr;
+};
+
+// One-sided (block) Jacobi SVD of the column-major matrix X (rows x cols,
+// FP64), in place.  On exit the columns are orthogonal; sigma (sorted
+// descending), perm (column index of each sorted value) and the truncation
+// result are written to device memory.
 void launch_jacobi_svd(c128* X, int rows, int cols, double* sigma, int* perm, SvdResult* res,
-                       int chi_max, double eps, int rank_cap, hipStream_t s);
+                       int chi_max, double eps, int rank_cap, hipStream_t s, JacobiWork& w);
 
 // Householder QR with LAPACK's zgeqr2/zung2r conventions (real beta, beta =
 // -sign(Re alpha)*norm) of the column-major matrix M (rows x cols), in place:

@@ -24,11 +24,15 @@ struct Error : std::runtime_error {
   Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
 };
 
+// On failure the runtime's sticky last-error is cleared before throwing, so
+// that the error is reported once, through this library's return code.
 #define QCA_HIP(call)                                                              \
   do {                                                                             \
     hipError_t e_ = (call);                                                        \
-    if (e_ != hipSuccess)                                                          \
+    if (e_ != hipSuccess) {                                                        \
+      (void)hipGetLastError();                                                     \
       throw ::qca::Error(-3, std::string(#call " failed: ") + hipGetErrorString(e_)); \
+    }                                                                              \
   } while (0)
 
 #define QCA_CHECK(cond, msg)                                                       \

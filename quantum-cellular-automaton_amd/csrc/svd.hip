@@ -1,13 +1,18 @@
 // FP64 factorisations of the TDVP split (gfx950).
 //
-// jacobi_svd: one-sided (Hestenes) Jacobi on the columns of a column-major
-// matrix, one workgroup of 16 wave64s.  Column pairs follow the round-robin
-// tournament so that every round updates n/2 disjoint pairs in parallel, one
-// pair per wave at a time, with a wave-shuffle reduction for the 2x2 Gram
-// entries.  Jacobi keeps small singular values accurate to working precision
-// relative to the largest one, which the 1e-10 truncation tail test needs.
-// The kernel also sorts the singular values and evaluates the truncation rank
-// (first i with ||s[i:]|| < eps, capped; reference algorithms/tdvp.py:289-292).
+// jacobi_svd: block one-sided (Hestenes) Jacobi on the columns of a
+// column-major matrix in one cooperative launch.  The columns are cut into
+// blocks sized so that a pair of blocks fits in 128 KiB of LDS; in every round
+// each workgroup loads one block pair, applies a full inner sweep of column
+// rotations in LDS (round-robin pairs, one pair per wave64, wave-shuffle
+// reductions for the 2x2 Gram entries), and writes it back; rounds follow the
+// round-robin tournament over blocks and are separated by grid barriers.
+// Jacobi keeps singular values accurate to working precision relative to the
+// largest one, which the 1e-10 truncation tail test needs; columns below
+// 1e-15 of the matrix norm are left alone (their contribution to any tail is
+// < 1e-14).  The kernel also sorts the singular values and evaluates the
+// truncation rank (first i with ||s[i:]|| < eps, capped; reference
+// algorithms/tdvp.py:289-292).
 //
 // householder_qr: zgeqr2 + zung2r semantics (reference tensor_networks/mps.py:86
 // calls numpy.linalg.qr -> LAPACK), including LAPACK's zlarfg sign convention
@@ -21,7 +26,8 @@ namespace {
 constexpr int JAC_THREADS = 1024;
 constexpr int JAC_WAVES = JAC_THREADS / 64;
 constexpr int JAC_MAXCOLS = 512;
-constexpr int JAC_MAXR = 8;  // rows per lane (rows <= 512)
+constexpr int JAC_MAXR = 8;               // rows per lane (rows <= 512)
+constexpr int JAC_LDS_BYTES = 128 * 1024;  // one block pair of columns
 
 __device__ __forceinline__ double wsum(double v) {
 #pragma unroll
@@ -29,115 +35,169 @@ __device__ __forceinline__ double wsum(double v) {
   return v;
 }
 
-__global__ __launch_bounds__(JAC_THREADS) void jacobi_svd_kernel(c128* __restrict__ X, int rows,
-                                                                 int cols, double* __restrict__ sigma,
-                                                                 int* __restrict__ perm,
-                                                                 SvdResult* __restrict__ res,
-                                                                 int chi_max, double eps,
-                                                                 int rank_cap) {
-  __shared__ double nrm[JAC_MAXCOLS];
-  __shared__ double skey[JAC_MAXCOLS];
-  __shared__ int sidx[JAC_MAXCOLS];
-  __shared__ int rot_count;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int nr = (rows + 63) / 64;  // rows handled per lane
-  const double tol = 1.1102230246251565e-16 * sqrt((double)rows);
+// Position t of round r in the round-robin tournament over n (even) players.
+__device__ __forceinline__ int rr_player(int t, int r, int n) {
+  return (t == 0) ? 0 : ((t - 1 + r) % (n - 1)) + 1;
+}
 
-  // exact squared column norms
-  for (int c = wave; c < cols; c += JAC_WAVES) {
-    const c128* x = X + (long)c * rows;
-    double a = 0;
-    for (int r = lane; r < rows; r += 64) a += x[r].re * x[r].re + x[r].im * x[r].im;
-    a = wsum(a);
-    if (lane == 0) nrm[c] = a;
+// One Hestenes rotation of the column pair (xi, xj) held in LDS (rows long).
+// Returns 1 when a rotation was applied.
+__device__ int jacobi_pair(c128* xi, c128* xj, int rows, int lane, double tol, double negl) {
+  double ir[JAC_MAXR], ii[JAC_MAXR], jr[JAC_MAXR], ji[JAC_MAXR];
+  double a = 0, b = 0, gr = 0, gi = 0;
+#pragma unroll
+  for (int q = 0; q < JAC_MAXR; ++q) {
+    const int r = lane + 64 * q;
+    if (r < rows) {
+      const c128 u = xi[r], v = xj[r];
+      ir[q] = u.re; ii[q] = u.im; jr[q] = v.re; ji[q] = v.im;
+    } else {
+      ir[q] = ii[q] = jr[q] = ji[q] = 0.0;
+    }
+    a += ir[q] * ir[q] + ii[q] * ii[q];
+    b += jr[q] * jr[q] + ji[q] * ji[q];
+    gr += ir[q] * jr[q] + ii[q] * ji[q];  // g = x_i^H x_j
+    gi += ir[q] * ji[q] - ii[q] * jr[q];
+  }
+  a = wsum(a); b = wsum(b); gr = wsum(gr); gi = wsum(gi);
+  const double ag = sqrt(gr * gr + gi * gi);
+  // converged pair, or both columns below the rounding level of the matrix
+  if (ag <= tol * sqrt(a) * sqrt(b) || ag < 1e-280 || fmax(a, b) < negl) return 0;
+  const double zeta = (b - a) / (2.0 * ag);
+  double t;
+  if (fabs(zeta) > 1e100) t = 0.5 / zeta;
+  else t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+  const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+  const double er = gr / ag, ei = gi / ag;
+  // x_i' = c x_i - s conj(e) x_j ;  x_j' = s e x_i + c x_j
+#pragma unroll
+  for (int q = 0; q < JAC_MAXR; ++q) {
+    const int r = lane + 64 * q;
+    if (r < rows) {
+      const double ce_r = er * jr[q] + ei * ji[q], ce_i = er * ji[q] - ei * jr[q];
+      const double e_r = er * ir[q] - ei * ii[q], e_i = er * ii[q] + ei * ir[q];
+      xi[r] = c128{c * ir[q] - s * ce_r, c * ii[q] - s * ce_i};
+      xj[r] = c128{s * e_r + c * jr[q], s * e_i + c * ji[q]};
+    }
+  }
+  return 1;
+}
+
+// Grid barrier for a cooperative launch (all workgroups co-resident): every
+// workgroup adds 1 to bar[0] and waits until the count reaches the target of
+// this barrier instance (k-th barrier: k * gridDim.x).  Every thread fences at
+// device scope on both sides, so global stores before the barrier are visible
+// to, and stale cache lines are not read by, any thread after it.
+__device__ void grid_barrier(unsigned int* bar, unsigned int& target) {
+  target += gridDim.x;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(bar, 1u);
+    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(1);
+    }
   }
   __syncthreads();
+  __threadfence();
+}
 
-  const int n2 = cols + (cols & 1);
+__global__ void zero_words(unsigned int* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+// Block one-sided Jacobi.  gridDim.x = nb/2 workgroups, each sweeping one pair
+// of column blocks (bs columns each) in LDS per round; rounds follow the
+// round-robin tournament over the nb blocks, separated by grid barriers.  The
+// loop ends after a sweep without rotations (all workgroups read the same
+// counter after a barrier) or after max_sweeps.  Workgroup 0 then sorts the
+// column norms and evaluates the truncation rank.
+__global__ __launch_bounds__(JAC_THREADS) void block_jacobi_kernel(
+    c128* __restrict__ X, int rows, int cols, int bs, int nb, int max_sweeps, double tol,
+    unsigned int* __restrict__ bar, int* __restrict__ rotc, double* __restrict__ fro,
+    double* __restrict__ sigma, int* __restrict__ perm, SvdResult* __restrict__ res, int chi_max,
+    double eps, int rank_cap) {
+  extern __shared__ unsigned char jac_smem[];
+  c128* lds = reinterpret_cast<c128*>(jac_smem);  // [2 bs][rows]
+  __shared__ int local_rot;
+  __shared__ double red[JAC_WAVES];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wg = blockIdx.x, nwg = gridDim.x;
+  unsigned int target = 0;
+
+  // Frobenius norm^2 of X (for the negligible-column threshold)
+  double part = 0;
+  for (long e = (long)wg * JAC_THREADS + tid; e < (long)rows * cols; e += (long)nwg * JAC_THREADS)
+    part += X[e].re * X[e].re + X[e].im * X[e].im;
+  part = wsum(part);
+  if (lane == 0) red[wave] = part;
+  __syncthreads();
+  if (tid == 0) {
+    double sacc = 0;
+    for (int w = 0; w < JAC_WAVES; ++w) sacc += red[w];
+    atomicAdd(fro, sacc);
+  }
+  grid_barrier(bar, target);
+  const double negl = 1e-30 * __hip_atomic_load(fro, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+
+  const int lc = 2 * bs;  // local columns
   int sweep = 0, converged = 0;
-  for (sweep = 0; sweep < 40; ++sweep) {
-    if (tid == 0) rot_count = 0;
-    __syncthreads();
-    for (int round = 0; round < n2 - 1; ++round) {
-      for (int k = wave; k < n2 / 2; k += JAC_WAVES) {
-        const int t1 = k, t2 = n2 - 1 - k;
-        const int i = (t1 == 0) ? 0 : ((t1 - 1 + round) % (n2 - 1)) + 1;
-        const int j = ((t2 - 1 + round) % (n2 - 1)) + 1;
-        if (i >= cols || j >= cols) continue;
-        c128* xi = X + (long)i * rows;
-        c128* xj = X + (long)j * rows;
-        double ir[JAC_MAXR], ii[JAC_MAXR], jr[JAC_MAXR], ji[JAC_MAXR];
-        double gr = 0, gi = 0;
-#pragma unroll
-        for (int q = 0; q < JAC_MAXR; ++q) {
-          const int r = lane + 64 * q;
-          if (q < nr && r < rows) {
-            ir[q] = xi[r].re; ii[q] = xi[r].im;
-            jr[q] = xj[r].re; ji[q] = xj[r].im;
-          } else {
-            ir[q] = ii[q] = jr[q] = ji[q] = 0.0;
-          }
-          // g = x_i^H x_j
-          gr += ir[q] * jr[q] + ii[q] * ji[q];
-          gi += ir[q] * ji[q] - ii[q] * jr[q];
-        }
-        gr = wsum(gr);
-        gi = wsum(gi);
-        const double a = nrm[i], b = nrm[j];
-        const double ag = sqrt(gr * gr + gi * gi);
-        if (ag <= tol * sqrt(a) * sqrt(b) || ag < 1e-280) continue;
-        const double zeta = (b - a) / (2.0 * ag);
-        double t;
-        if (fabs(zeta) > 1e100) t = 0.5 / zeta;
-        else t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-        const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
-        const double er = gr / ag, ei = gi / ag;
-        // x_i' = c x_i - s conj(e) x_j ;  x_j' = s e x_i + c x_j
-        double na = 0, nb = 0;
-#pragma unroll
-        for (int q = 0; q < JAC_MAXR; ++q) {
-          const double ce_r = er * jr[q] + ei * ji[q];   // conj(e) * x_j
-          const double ce_i = er * ji[q] - ei * jr[q];
-          const double e_r = er * ir[q] - ei * ii[q];    // e * x_i
-          const double e_i = er * ii[q] + ei * ir[q];
-          const double nir = c * ir[q] - s * ce_r, nii = c * ii[q] - s * ce_i;
-          const double njr = s * e_r + c * jr[q], nji = s * e_i + c * ji[q];
-          ir[q] = nir; ii[q] = nii; jr[q] = njr; ji[q] = nji;
-          na += nir * nir + nii * nii;
-          nb += njr * njr + nji * nji;
-        }
-#pragma unroll
-        for (int q = 0; q < JAC_MAXR; ++q) {
-          const int r = lane + 64 * q;
-          if (q < nr && r < rows) {
-            xi[r] = c128{ir[q], ii[q]};
-            xj[r] = c128{jr[q], ji[q]};
-          }
-        }
-        na = wsum(na);
-        nb = wsum(nb);
-        if (lane == 0) {
-          nrm[i] = na;
-          nrm[j] = nb;
-          atomicAdd(&rot_count, 1);
-        }
+  for (sweep = 0; sweep < max_sweeps; ++sweep) {
+    if (tid == 0) local_rot = 0;
+    for (int round = 0; round < nb - 1; ++round) {
+      const int bi = rr_player(wg, round, nb), bj = rr_player(nb - 1 - wg, round, nb);
+      // stage the two blocks (zero columns beyond cols)
+      for (int e = tid; e < lc * rows; e += JAC_THREADS) {
+        const int c = e / rows, r = e - c * rows;
+        const int gc = (c < bs ? bi * bs + c : bj * bs + (c - bs));
+        lds[e] = (gc < cols) ? X[(long)gc * rows + r] : c128{0.0, 0.0};
       }
       __syncthreads();
+      // one inner sweep over the 2 bs local columns
+      for (int ir = 0; ir < lc - 1; ++ir) {
+        for (int pr = wave; pr < bs; pr += JAC_WAVES) {
+          const int ci = rr_player(pr, ir, lc), cj = rr_player(lc - 1 - pr, ir, lc);
+          const int n = jacobi_pair(lds + (long)ci * rows, lds + (long)cj * rows, rows, lane, tol, negl);
+          if (n && lane == 0) atomicAdd(&local_rot, 1);
+        }
+        __syncthreads();
+      }
+      for (int e = tid; e < lc * rows; e += JAC_THREADS) {
+        const int c = e / rows, r = e - c * rows;
+        const int gc = (c < bs ? bi * bs + c : bj * bs + (c - bs));
+        if (gc < cols) X[(long)gc * rows + r] = lds[e];
+      }
+      grid_barrier(bar, target);
     }
-    const int rc = rot_count;
-    __syncthreads();
-    if (rc == 0) {
+    if (tid == 0) atomicAdd(&rotc[sweep & 1], local_rot);
+    grid_barrier(bar, target);
+    const int total = __hip_atomic_load(&rotc[sweep & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (wg == 0 && tid == 0) rotc[(sweep + 1) & 1] = 0;
+    grid_barrier(bar, target);
+    if (total == 0) {
       converged = 1;
       break;
     }
   }
+  if (wg != 0) return;
 
-  // sort singular values descending (bitonic over the next power of two)
+  // ---- workgroup 0: column norms, descending sort, truncation rank
+  double* skey = reinterpret_cast<double*>(jac_smem);            // reuse LDS
+  int* sidx = reinterpret_cast<int*>(skey + JAC_MAXCOLS);
   int np2 = 1;
   while (np2 < cols) np2 <<= 1;
-  for (int e = tid; e < np2; e += JAC_THREADS) {
-    skey[e] = (e < cols) ? sqrt(nrm[e]) : -1.0;
-    sidx[e] = e;
+  for (int c = wave; c < np2; c += JAC_WAVES) {
+    double a = 0;
+    if (c < cols)
+      for (int r = lane; r < rows; r += 64) {
+        const c128 v = X[(long)c * rows + r];
+        a += v.re * v.re + v.im * v.im;
+      }
+    a = wsum(a);
+    if (lane == 0) {
+      skey[c] = (c < cols) ? sqrt(a) : -1.0;
+      sidx[c] = c;
+    }
   }
   __syncthreads();
   for (int size = 2; size <= np2; size <<= 1) {
@@ -164,24 +224,19 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_svd_kernel(c128* __restric
   }
   __syncthreads();
   if (tid == 0) {
-    // tail norms over the min(rows, cols) values a reduced SVD returns
+    // first i (< cap) with ||s[i:]|| < eps over the min(rows, cols) values a
+    // reduced SVD returns; the tail norms form a non-increasing sequence
     const int ns = rank_cap;
     const int cap = min(chi_max, ns);
     int k = cap;
     double suffix = 0;
-    // suffix[i] = sum_{j>=i} s_j^2, scanned from the end; the first i (< cap)
-    // whose tail norm is below eps is the rank.
-    double tails_cap = 0;
     for (int e = ns - 1; e >= 0; --e) {
       suffix += skey[e] * skey[e];
-      if (e < cap && sqrt(suffix) < eps) k = e;  // keeps the smallest such e
-      if (e == cap) tails_cap = suffix;
+      if (e < cap && sqrt(suffix) < eps) k = e;
     }
-    double kept = 0;
+    double kept = 0, tail = 0;
     for (int e = 0; e < k; ++e) kept += skey[e] * skey[e];
-    double tail = 0;
     for (int e = k; e < ns; ++e) tail += skey[e] * skey[e];
-    (void)tails_cap;
     res->k = k;
     res->nsweeps = sweep + (converged ? 1 : 0);
     res->converged = converged;
@@ -308,12 +363,46 @@ __global__ __launch_bounds__(QR_THREADS) void householder_qr_kernel(c128* __rest
 }  // namespace
 
 void launch_jacobi_svd(c128* X, int rows, int cols, double* sigma, int* perm, SvdResult* res,
-                       int chi_max, double eps, int rank_cap, hipStream_t s) {
+                       int chi_max, double eps, int rank_cap, hipStream_t s, JacobiWork& w) {
   QCA_CHECK(rows >= 1 && rows <= 64 * JAC_MAXR, "SVD rows out of range (1..512)");
   QCA_CHECK(cols >= 1 && cols <= JAC_MAXCOLS, "SVD cols out of range (1..512)");
-  hipLaunchKernelGGL(jacobi_svd_kernel, dim3(1), dim3(JAC_THREADS), 0, s, X, rows, cols, sigma, perm,
-                     res, chi_max, eps, rank_cap);
-  QCA_HIP(hipGetLastError());
+  // block size: a pair of blocks (2 bs columns of `rows` complex doubles) fits
+  // in 128 KiB of LDS, and is no wider than the matrix needs
+  int bs = 1;
+  while (bs * 2 <= 64 && 2L * (bs * 2) * rows * (long)sizeof(c128) <= JAC_LDS_BYTES &&
+         2 * bs < cols)
+    bs *= 2;
+  int nb = ceil_div(cols, bs);
+  if (nb < 2) nb = 2;
+  if (nb & 1) ++nb;
+  const int nwg = nb / 2;
+  const size_t shm = std::max<size_t>(2UL * bs * rows * sizeof(c128),
+                                      JAC_MAXCOLS * (sizeof(double) + sizeof(int)));
+  static size_t attr_bytes = 0;  // dynamic LDS opt-in (static + dynamic <= 160 KiB)
+  if (shm > attr_bytes) {
+    QCA_HIP(hipFuncSetAttribute((const void*)block_jacobi_kernel,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    attr_bytes = shm;
+  }
+  hipLaunchKernelGGL(zero_words, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned int*>(w.sync), 16);
+  int max_sweeps = 60;
+  double tol = 1.1102230246251565e-16 * sqrt((double)rows);
+  unsigned int* bar = reinterpret_cast<unsigned int*>(w.sync);
+  int* rotc = reinterpret_cast<int*>(w.sync) + 4;
+  double* fro = reinterpret_cast<double*>(reinterpret_cast<char*>(w.sync) + 32);
+  void* args[] = {&X, &rows, &cols, &bs, &nb, &max_sweeps, &tol, &bar, &rotc, &fro,
+                  &sigma, &perm, &res, &chi_max, &eps, &rank_cap};
+  if (nwg == 1) {
+    // one workgroup: the grid barrier reduces to a workgroup barrier
+    hipLaunchKernelGGL(block_jacobi_kernel, dim3(1), dim3(JAC_THREADS), shm, s, X, rows, cols, bs, nb,
+                       max_sweeps, tol, bar, rotc, fro, sigma, perm, res, chi_max, eps, rank_cap);
+    QCA_HIP(hipGetLastError());
+  } else {
+    // several workgroups wait on each other: cooperative launch guarantees
+    // they are co-resident (or fails instead of hanging)
+    QCA_HIP(hipLaunchCooperativeKernel((const void*)block_jacobi_kernel, dim3(nwg),
+                                       dim3(JAC_THREADS), args, (unsigned int)shm, s));
+  }
 }
 
 void launch_householder_qr(c128* M, int rows, int cols, c128* tau, c128* Q, int qcols,

@@ -1,0 +1,11 @@
+"""Command-line entry point (reference main.py:5-9), GPU time evolution."""
+from parameters import Parser
+from quantum_game import QuantumGame
+
+
+def main() -> None:
+    QuantumGame(Parser.instance()).start()
+
+
+if __name__ == "__main__":
+    main()

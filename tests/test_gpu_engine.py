@@ -192,3 +192,29 @@ def test_exact_class_matches_oracle(gpu):
             ex.measure(pop[r], np.zeros(n), np.zeros(n), np.zeros(n + 1))
             ex.do_time_step()
         assert np.abs(pop - g[f"c{c}_population"]).max() < 1e-10
+
+
+def test_driver_cadence_and_files(gpu, tmp_path):
+    """quantum_game: measure-then-step cadence, CSV rows [:plot_step] (the
+    reference's off-by-one), npz checkpoint readable as an initial state."""
+    from parameters import Parser
+    from quantum_game import QuantumGame
+    from tensor_networks import MPS
+    args = Parser(["--num-cells", "10", "--algorithm", "2tdvp", "--step-size", "0.1", "--num-steps", "12",
+                   "--plotting-frequency", "5", "--max-bond-dim", "16", "--svd-epsilon", "1e-10",
+                   "--initial-states", "single"])
+    assert args.plot_step_interval == 2 and args.plot_steps == 6
+    out = QuantumGame(args).start(data_dir=str(tmp_path))[0]
+    name = out["name"]
+    csv = np.loadtxt(tmp_path / f"{name}-population.csv", delimiter=",")
+    assert csv.shape == (5, 10)  # rows [:5] written at the last plot step
+    assert np.allclose(csv, out["population"][:5])
+    ck = MPS.from_file(str(tmp_path / f"{name}.npz"))
+    assert ck.is_valid_mps() and abs(np.linalg.norm(ck.as_vector()) - 1) < 1e-10
+    # the same run step by step through the oracle
+    pl = [0.0] * 10
+    pl[5] = 1.0
+    o = OT.TDVPOracle(OM.product_state(pl), AU.mpo_tensors(10, 1, range(1, 2)), "2tdvp", 0.1, 16, 1e-10,
+                      gauge="consistent", expm="lanczos")
+    ref = OT.run_trajectory(o, 12, 2)
+    assert np.abs(out["population"] - ref["population"]).max() < 1e-10
