@@ -58,7 +58,7 @@ def mpo():
     return MPO.hamiltonian_from_rules(Rules(N, IV, DIST, False))
 
 
-def heff2_roofline(torch, prec: int, reps: int = 20):
+def heff2_roofline(torch, prec: int, reps: int = 50):
     """Time the 2-site H_eff application at chi=128, D=13 with HIP events."""
     import qca_native as qn
     from tensor_networks import MPO
@@ -80,17 +80,18 @@ def heff2_roofline(torch, prec: int, reps: int = 20):
     for _ in range(3):
         qn.call("qca_heff_apply", prec, 0, qn.ptr(x), qn.ptr(y), qn.ptr(L), qn.ptr(R), Wh.ctypes.data,
                 D, D, chi, chi, sp)
+    torch.cuda.synchronize()
+    # qca_heff_apply only queues its kernels (the sparse MPO is uploaded on the
+    # first call), so `reps` back-to-back applications between two events on
+    # that stream give the average device time of one application.
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    # qca_heff_apply synchronises its stream after each call; time each call
-    # between events recorded on that same stream and average.
-    total = 0.0
+    e0.record(stream)
     for _ in range(reps):
-        e0.record(stream)
         qn.call("qca_heff_apply", prec, 0, qn.ptr(x), qn.ptr(y), qn.ptr(L), qn.ptr(R), Wh.ctypes.data,
                 D, D, chi, chi, sp)
-        e1.record(stream)
-        e1.synchronize()
-        total += e0.elapsed_time(e1)
+    e1.record(stream)
+    e1.synchronize()
+    total = e0.elapsed_time(e1)
     ms = total / reps
     flops = 8.0 * (4.0 * D * chi * chi * (2 * chi) + 16.0 * D * D * chi * chi)
     return ms, flops
@@ -131,6 +132,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="run only the H_eff micro-benchmark and print its roofline object "
+                         "(the command the kernel profiles are taken with)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: every box in this pool has a single MI355X, and the "
@@ -141,6 +145,14 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("no HIP device visible", file=sys.stderr)
         return 2
+    if a.roofline_only:
+        ms32, fl = heff2_roofline(torch, 1)
+        ms64, _ = heff2_roofline(torch, 0)
+        print(json.dumps({"roofline": {"mixed_ms_per_launch": ms32, "fp64_ms_per_launch": ms64,
+                                       "flops_per_launch": fl,
+                                       "mixed_tflops": fl / (ms32 * 1e-3) / 1e12,
+                                       "fp64_tflops": fl / (ms64 * 1e-3) / 1e12}}), flush=True)
+        return 0
     Ws = mpo().W
     psi = initial_state()
     # W untimed steps of the plain engine grow the bonds towards chi_max; the
@@ -210,6 +222,9 @@ def main() -> int:
         "max_abs_dn_mixed_vs_fp64": maxdiff,
         "gpu_ms_per_step": {k: v / a.steps for k, v in gpu_ms.items()},
         "matvecs_per_step": {k: v["matvecs"] / a.steps for k, v in flops.items()},
+        "phase_gpu_ms_per_step": {k: {"svd": v["svd_ms"] / a.steps, "krylov": v["krylov_ms"] / a.steps}
+                                  for k, v in flops.items()},
+        "svd_sweeps_per_call": {k: v["svd_sweeps"] / max(1, v["svd_calls"]) for k, v in flops.items()},
         "contraction_tflops_per_s": {k: flops[k]["flops"] / times[k] / 1e12 for k in flops},
         "roofline": {"bound": "mfma", "kernel": "2-site H_eff application (mixed, complex64 on FP32 MFMA)",
                      "achieved": ach32, "peak": 157.3, "unit": "TFLOP/s", "frac": ach32 / 157.3,

@@ -20,10 +20,13 @@
  *    (ket, MPO, bra), two-site Theta[(p1 p2)][a][c] (4, chi_l, chi_r).
  *  - "device" pointers are HIP device memory owned by the caller (for example
  *    torch tensor data_ptr()); "host" pointers are ordinary host memory.
- *    Primitive calls are ordered on the given stream (NULL = default stream)
- *    and return after the stream has drained when they report a host-side
- *    result (rank, iteration count).  Primitives on one device share the
- *    library's workspace: calls that overlap on one device race.
+ *    Primitive calls are ordered on the given stream (NULL = default stream).
+ *    qca_heff_apply and qca_env_update return once their kernels are queued;
+ *    calls that report a host-side result (rank, iteration count) return after
+ *    the stream has drained.  The sparse MPO form of each distinct W is built
+ *    and uploaded on first use and kept until qca_release_workspace.
+ *    Primitives on one device share the library's workspace: calls that
+ *    overlap on one device race.
  *  - precision: 0 = complex128 contractions (plain FP64 path),
  *               1 = complex64 contractions on FP32 matrix cores (mixed path;
  *                   Krylov vectors, SVD and QR stay FP64).
@@ -69,8 +72,10 @@ typedef struct qca_engine_params {
 
 typedef struct qca_stats {
   long matvecs, matvec_2site, matvec_1site, matvec_bond;
-  long site_updates, svd_sweeps, lanczos_limit_hits;
+  long site_updates, svd_sweeps, lanczos_limit_hits, svd_calls;
   double flops;      /* algorithmic contraction flops executed */
+  double svd_ms;     /* device time of the 2-site SVD splits */
+  double krylov_ms;  /* device time of the Krylov exponentials */
 } qca_stats;
 
 /* W[i]: host, complex128 (2,2,Wdl[i],Wdr[i]); A[i]: host, complex128

@@ -48,9 +48,38 @@ struct Scratch {
     bufs[name] = {p, bytes};
     return p;
   }
+  // Device copies of the sparse MPO forms passed to the primitives, keyed by
+  // their defining bytes, so that repeated calls upload nothing and stay
+  // stream-asynchronous.
+  std::map<std::string, qca::WSparsePair> mpos;
+  qca::WSparsePair& mpo(const double* W_host, int dq, int Dl, int Dr, bool backward, int nin) {
+    const size_t bytes = sizeof(qca::c128) * (size_t)dq * dq * Dl * Dr;
+    std::string key(reinterpret_cast<const char*>(W_host), bytes);
+    key += std::to_string(dq) + "/" + std::to_string(Dl) + "/" + std::to_string(Dr) + "/" +
+           std::to_string((int)backward) + "/" + std::to_string(nin);
+    auto it = mpos.find(key);
+    if (it != mpos.end()) return it->second;
+    if (mpos.size() >= 64) release_mpos();
+    qca::WSparsePair wp;
+    try {
+      wp.upload(qca::build_sparse(reinterpret_cast<const qca::c128*>(W_host), dq, Dl, Dr, backward),
+                nin);
+    } catch (...) {
+      wp.release();
+      throw;
+    }
+    return mpos.emplace(std::move(key), std::move(wp)).first->second;
+  }
+  void release_mpos() {
+    QCA_HIP(hipDeviceSynchronize());
+    for (auto& kv : mpos) kv.second.release();
+    mpos.clear();
+  }
   void release() {
     for (auto& kv : bufs) (void)hipFree(kv.second.first);
     bufs.clear();
+    for (auto& kv : mpos) kv.second.release();
+    mpos.clear();
   }
 };
 std::map<int, Scratch> g_scratch;
@@ -229,7 +258,10 @@ int qca_engine_stats(qca_engine* e, qca_stats* out) {
     out->site_updates = s.site_updates;
     out->svd_sweeps = s.svd_sweeps;
     out->lanczos_limit_hits = s.lanczos_limit_hits;
+    out->svd_calls = s.svd_calls;
     out->flops = s.flops;
+    out->svd_ms = s.svd_ms;
+    out->krylov_ms = s.krylov_ms;
   });
 }
 
@@ -262,26 +294,18 @@ int qca_heff_apply(int precision, int kind, const void* x, void* y, const void* 
     hipStream_t s = as_stream(stream);
     const int dq = kind == 0 ? 4 : (kind == 1 ? 2 : 1);
     const long tsize = (long)std::max(dq, 1) * std::max(Din, Dout) * chil * chir;
-    qca::WSparsePair wp;
-    if (kind != 2) wp.upload(qca::build_sparse(reinterpret_cast<const qca::c128*>(W_host), dq, Din, Dout, false), dq * Din);
-    try {
-      if (precision == 0) {
-        auto ws = contract_ws<double>(tsize, (long)dq * chil * chir);
-        qca::heff_apply<double>(kind, (const qca::c128*)x, (qca::c128*)y, (const qca::c128*)L,
-                                (const qca::c128*)R, kind == 2 ? nullptr : &wp.d64, Din, Dout, chil,
-                                chir, ws, nullptr, s);
-      } else {
-        auto ws = contract_ws<float>(tsize, (long)dq * chil * chir);
-        qca::heff_apply<float>(kind, (const qca::c64*)x, (qca::c64*)y, (const qca::c64*)L,
-                               (const qca::c64*)R, kind == 2 ? nullptr : &wp.d32, Din, Dout, chil,
-                               chir, ws, nullptr, s);
-      }
-      QCA_HIP(hipStreamSynchronize(s));
-    } catch (...) {
-      wp.release();
-      throw;
+    qca::WSparsePair* wp = kind == 2 ? nullptr : &scratch().mpo(W_host, dq, Din, Dout, false, dq * Din);
+    if (precision == 0) {
+      auto ws = contract_ws<double>(tsize, (long)dq * chil * chir);
+      qca::heff_apply<double>(kind, (const qca::c128*)x, (qca::c128*)y, (const qca::c128*)L,
+                              (const qca::c128*)R, wp ? &wp->d64 : nullptr, Din, Dout, chil, chir,
+                              ws, nullptr, s);
+    } else {
+      auto ws = contract_ws<float>(tsize, (long)dq * chil * chir);
+      qca::heff_apply<float>(kind, (const qca::c64*)x, (qca::c64*)y, (const qca::c64*)L,
+                             (const qca::c64*)R, wp ? &wp->d32 : nullptr, Din, Dout, chil, chir, ws,
+                             nullptr, s);
     }
-    wp.release();
   });
 }
 
@@ -295,28 +319,19 @@ int qca_env_update(int precision, int side, const void* A, const void* Eprev, vo
     QCA_CHECK(Dl >= 1 && Dr >= 1 && Dl <= 16 && Dr <= 16, "MPO bond out of range");
     hipStream_t s = as_stream(stream);
     const bool left = side == 0;
-    qca::WSparsePair wp;
-    wp.upload(qca::build_sparse(reinterpret_cast<const qca::c128*>(W_host), 2, Dl, Dr, !left),
-              left ? 2 * Dl : 2 * Dr);
+    qca::WSparsePair& wp = scratch().mpo(W_host, 2, Dl, Dr, !left, left ? 2 * Dl : 2 * Dr);
     const int Din = left ? Dl : Dr, Dout = left ? Dr : Dl;
     const long tsize = 2L * std::max(Dl, Dr) * chil * chir;
     const long osize = (long)std::max(chil, chir) * std::max(Dl, Dr) * std::max(chil, chir);
-    try {
-      if (precision == 0) {
-        auto ws = contract_ws<double>(tsize, osize);
-        qca::env_update<double>(left, (const qca::c128*)A, (const qca::c128*)Eprev,
-                                (qca::c128*)Enew, &wp.d64, Din, Dout, chil, chir, ws, s);
-      } else {
-        auto ws = contract_ws<float>(tsize, osize);
-        qca::env_update<float>(left, (const qca::c64*)A, (const qca::c64*)Eprev, (qca::c64*)Enew,
-                               &wp.d32, Din, Dout, chil, chir, ws, s);
-      }
-      QCA_HIP(hipStreamSynchronize(s));
-    } catch (...) {
-      wp.release();
-      throw;
+    if (precision == 0) {
+      auto ws = contract_ws<double>(tsize, osize);
+      qca::env_update<double>(left, (const qca::c128*)A, (const qca::c128*)Eprev, (qca::c128*)Enew,
+                              &wp.d64, Din, Dout, chil, chir, ws, s);
+    } else {
+      auto ws = contract_ws<float>(tsize, osize);
+      qca::env_update<float>(left, (const qca::c64*)A, (const qca::c64*)Eprev, (qca::c64*)Enew,
+                             &wp.d32, Din, Dout, chil, chir, ws, s);
     }
-    wp.release();
   });
 }
 
@@ -334,20 +349,19 @@ int qca_krylov_expm(int precision, int kind, void* x, const void* L, const void*
     const int dq = kind == 0 ? 4 : (kind == 1 ? 2 : 1);
     const long n = (long)dq * chil * chir;
     const long tsize = (long)dq * std::max(Din, Dout) * chil * chir;
-    qca::WSparsePair wp;
-    if (kind != 2) wp.upload(qca::build_sparse(reinterpret_cast<const qca::c128*>(W_host), dq, Din, Dout, false), dq * Din);
+    qca::WSparsePair* wp = kind == 2 ? nullptr : &scratch().mpo(W_host, dq, Din, Dout, false, dq * Din);
     qca::LanczosCtrl* host = nullptr;
     qca::LanczosWork lw = lanczos_ws(n, &host);
     const double tau = 1.5707963267948966 * delta;
     const double tl = tol > 0 ? tol : 1e-14;
-    try {
+    {
       int m = 0;
       if (precision == 0) {
         auto ws = contract_ws<double>(tsize, n);
         qca::c128* y = sbuf<qca::c128>("kr_y", n);
         m = krylov_loop((qca::c128*)x, n, lw, host, [&](int j, const qca::c128* vj) {
           qca::heff_apply<double>(kind, vj, y, (const qca::c128*)L, (const qca::c128*)R,
-                                  kind == 2 ? nullptr : &wp.d64, Din, Dout, chil, chir, ws,
+                                  wp ? &wp->d64 : nullptr, Din, Dout, chil, chir, ws,
                                   qca::lanczos_done_flag(lw), s);
           qca::lanczos_iteration_tail<double>(y, n, j, tau, tl, lw, s);
         }, s);
@@ -358,17 +372,13 @@ int qca_krylov_expm(int precision, int kind, void* x, const void* L, const void*
         m = krylov_loop((qca::c128*)x, n, lw, host, [&](int j, const qca::c128* vj) {
           qca::launch_convert_c128_c64(vj, x32, n, s, qca::lanczos_done_flag(lw));
           qca::heff_apply<float>(kind, x32, y, (const qca::c64*)L, (const qca::c64*)R,
-                                 kind == 2 ? nullptr : &wp.d32, Din, Dout, chil, chir, ws,
+                                 wp ? &wp->d32 : nullptr, Din, Dout, chil, chir, ws,
                                  qca::lanczos_done_flag(lw), s);
           qca::lanczos_iteration_tail<float>(y, n, j, tau, tl, lw, s);
         }, s);
       }
       if (iters) *iters = m;
-    } catch (...) {
-      wp.release();
-      throw;
     }
-    wp.release();
   });
 }
 
@@ -388,8 +398,10 @@ int qca_svd_split(const void* theta, void* Al, void* Ar, int chil, int chir, int
     int* perm = sbuf<int>("svd_perm", cols);
     qca::SvdResult* res = sbuf<qca::SvdResult>("svd_res", 1);
     qca::launch_theta_to_svd(th, X, chil, chir, right_sweep ? 0 : 1, s);
-    qca::JacobiWork jw;
-    jw.sync = sbuf<unsigned char>("svd_sync", 64);
+    static thread_local qca::JacobiWork jw;  // keeps the pinned word pair and the sweep count
+    jw.ws_bytes = qca::jacobi_workspace_bytes(rows, cols);
+    jw.ws = sbuf<unsigned char>("svd_ws", (long)jw.ws_bytes);
+    if (!jw.host) QCA_HIP(hipHostMalloc(&jw.host, 64));
     qca::launch_jacobi_svd(X, rows, cols, sigma, perm, res, chi_max, eps, rank_cap, s, jw);
     qca::SvdResult hres;
     QCA_HIP(hipMemcpyAsync(&hres, res, sizeof(hres), hipMemcpyDeviceToHost, s));

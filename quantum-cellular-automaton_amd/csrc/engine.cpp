@@ -152,9 +152,13 @@ Engine::Engine(int device, const EngineParams& p, const std::vector<const c128*>
   sigma_ = dmalloc<double>(2 * cmax);
   perm_ = dmalloc<int>(2 * cmax);
   svdres_ = dmalloc<SvdResult>(1);
-  jw_.sync = dmalloc<unsigned char>(64);
+  jw_.ws_bytes = jacobi_workspace_bytes(2 * cmax, 2 * cmax);
+  jw_.ws = dmalloc<unsigned char>((long)jw_.ws_bytes);
+  QCA_HIP(hipHostMalloc(&jw_.host, 64));
   QCA_HIP(hipHostMalloc((void**)&svdres_host_, sizeof(SvdResult)));
   QCA_HIP(hipHostMalloc((void**)&ctrl_host_, sizeof(LanczosCtrl)));
+  QCA_HIP(hipEventCreate(&ev0_));
+  QCA_HIP(hipEventCreate(&ev1_));
   lw_.max_n = nmax;
   lw_.max_blocks = lanczos_nblk(nmax);
   lw_.ctrl = dmalloc<LanczosCtrl>(1);
@@ -183,10 +187,19 @@ Engine::~Engine() {
   for (void* p : allocs_) (void)hipFree(p);
   if (svdres_host_) (void)hipHostFree(svdres_host_);
   if (ctrl_host_) (void)hipHostFree(ctrl_host_);
+  if (jw_.host) (void)hipHostFree(jw_.host);
+  if (ev0_) (void)hipEventDestroy(ev0_);
+  if (ev1_) (void)hipEventDestroy(ev1_);
   if (own_stream_ && s_) (void)hipStreamDestroy(s_);
 }
 
 void Engine::sync() { QCA_HIP(hipStreamSynchronize(s_)); }
+
+double Engine::elapsed_ms() {
+  float ms = 0.f;
+  QCA_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+  return ms;
+}
 
 template <>
 void Engine::gemm<double>(const GemmDesc& d, const c128* A, const c128* B, c128* C) {
@@ -242,6 +255,7 @@ void Engine::heff(int kind, int site, const cplx<T>* x, cplx<T>* y, int chil, in
 template <typename T>
 void Engine::lanczos_run(int kind, int site, c128* x, long n, int chil, int chir, double delta) {
   const double tau = kHalfPi * delta;
+  QCA_HIP(hipEventRecord(ev0_, s_));
   lanczos_begin(x, n, lw_, s_);
   int j = 0;
   int chunk = std::max(2, kpred_[kind]);
@@ -258,11 +272,13 @@ void Engine::lanczos_run(int kind, int site, c128* x, long n, int chil, int chir
       }
     }
     QCA_HIP(hipMemcpyAsync(ctrl_host_, lw_.ctrl, sizeof(LanczosCtrl), hipMemcpyDeviceToHost, s_));
+    QCA_HIP(hipEventRecord(ev1_, s_));
     sync();
     if (ctrl_host_->done) break;
     QCA_CHECK(j < lw_.max_m - 2, "Krylov exponential did not converge");
     chunk = 2;
   }
+  stats_.krylov_ms += elapsed_ms();
   const int m = ctrl_host_->m;
   kpred_[kind] = std::max(2, m);
   if (ctrl_host_->hit_limit) stats_.lanczos_limit_hits++;
@@ -327,10 +343,14 @@ int Engine::split_theta(int s, bool right_sweep, int chil, int chir) {
   const int rows = right_sweep ? 2 * chil : 2 * chir;
   const int cols = right_sweep ? 2 * chir : 2 * chil;
   const int rank_cap = std::min(2 * chil, 2 * chir);
+  QCA_HIP(hipEventRecord(ev0_, s_));
   launch_theta_to_svd(theta_, svdX_, chil, chir, right_sweep ? 0 : 1, s_);
   launch_jacobi_svd(svdX_, rows, cols, sigma_, perm_, svdres_, p_.chi_max, p_.eps, rank_cap, s_, jw_);
   QCA_HIP(hipMemcpyAsync(svdres_host_, svdres_, sizeof(SvdResult), hipMemcpyDeviceToHost, s_));
+  QCA_HIP(hipEventRecord(ev1_, s_));
   sync();
+  stats_.svd_ms += elapsed_ms();
+  stats_.svd_calls++;
   const int k = svdres_host_->k;
   stats_.svd_sweeps += svdres_host_->nsweeps;
   QCA_CHECK(k >= 1, "SVD truncation produced an empty bond");

@@ -28,7 +28,10 @@ struct StepStats {
   long site_updates = 0;
   long svd_sweeps = 0;
   long lanczos_limit_hits = 0;
+  long svd_calls = 0;
   double flops = 0;        // algorithmic contraction flops
+  double svd_ms = 0;       // device time of the SVDs (layout change to truncation rank)
+  double krylov_ms = 0;    // device time of the Krylov exponentials
 };
 
 class Engine {
@@ -113,6 +116,8 @@ class Engine {
   c128 *measE_ = nullptr, *measT_ = nullptr, *rho_ = nullptr;
   int kpred_[3] = {8, 8, 8};
   StepStats stats_;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;  // phase timers
+  double elapsed_ms();                         // ev0_ -> ev1_, both complete
 
   template <typename T> T* dmalloc(long n);
 };

@@ -560,12 +560,7 @@ void launch_wmix(const cplx<T>* Tin, long in_stride, long nxc, int nin, cplx<T>*
   QCA_CHECK(nin <= 64, "too many input channels in MPO mixing");
   if (nxc == 0) return;
   const size_t shm = ((size_t)nin * WMIX_XC + (size_t)WMIX_XC * Dout) * sizeof(cplx<T>);
-  static size_t attr_bytes = 0;  // dynamic LDS opt-in per instantiation
-  if (shm > attr_bytes) {
-    QCA_HIP(hipFuncSetAttribute((const void*)wmix_kernel<T>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    attr_bytes = shm;
-  }
+  ensure_dynamic_lds((const void*)wmix_kernel<T>, shm);
   hipLaunchKernelGGL(wmix_kernel<T>, dim3(ceil_div(nxc, WMIX_XC)), dim3(WMIX_XC), shm, s, Tin,
                      in_stride, nxc, nin, U, nq, Dout, w.rowptr, w.col, w.val, skip);
   QCA_HIP(hipGetLastError());

@@ -91,10 +91,24 @@ struct SvdResult {
   double tail;        // ||s[k:]||
 };
 
-// Device scratch of the Jacobi SVD: 64 bytes of barrier / counter words.
+// Workspace of the Jacobi SVD: device memory of at least
+// jacobi_workspace_bytes(rows, cols) (control block, partial Grams, block
+// rotations), a pinned host word pair, and the sweep count of the previous call
+// (the number of sweeps launched before the first convergence check).
 struct JacobiWork {
-  void* sync = nullptr;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  void* host = nullptr;
+  int pred_sweeps = 8;
 };
+
+struct JacobiLayout {
+  int bs = 1, nb = 2, kspl = 1;
+  long gpart_elems = 0, j_elems = 0;
+  size_t bytes = 0;
+};
+JacobiLayout jacobi_layout(int rows, int cols);
+size_t jacobi_workspace_bytes(int max_rows, int max_cols);
 
 // One-sided (block) Jacobi SVD of the column-major matrix X (rows x cols,
 // FP64), in place.  On exit the columns are orthogonal; sigma (sorted

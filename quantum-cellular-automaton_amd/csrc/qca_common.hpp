@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
+#include <mutex>
+#include <utility>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -41,6 +44,21 @@ struct Error : std::runtime_error {
   } while (0)
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+// Raises the dynamic-LDS limit of kernel `fn` on the current device to at
+// least `bytes` (once per kernel, device and size; safe from several threads).
+inline void ensure_dynamic_lds(const void* fn, size_t bytes) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, size_t> granted;
+  int dev = 0;
+  QCA_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  size_t& g = granted[{fn, dev}];
+  if (bytes > g) {
+    QCA_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    g = bytes;
+  }
+}
 
 // ---------------------------------------------------------------- GEMM descriptor
 // C[b](m,n) = alpha * sum_k opA(A)[b](m,k) * opB(B)[b](k,n) + beta * C[b](m,n)

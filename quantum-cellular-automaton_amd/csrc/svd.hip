@@ -1,17 +1,24 @@
 // FP64 factorisations of the TDVP split (gfx950).
 //
 // jacobi_svd: block one-sided (Hestenes) Jacobi on the columns of a
-// column-major matrix in one cooperative launch.  The columns are cut into
-// blocks sized so that a pair of blocks fits in 128 KiB of LDS; in every round
-// each workgroup loads one block pair, applies a full inner sweep of column
-// rotations in LDS (round-robin pairs, one pair per wave64, wave-shuffle
-// reductions for the 2x2 Gram entries), and writes it back; rounds follow the
-// round-robin tournament over blocks and are separated by grid barriers.
-// Jacobi keeps singular values accurate to working precision relative to the
-// largest one, which the 1e-10 truncation tail test needs; columns below
-// 1e-15 of the matrix norm are left alone (their contribution to any tail is
-// < 1e-14).  The kernel also sorts the singular values and evaluates the
-// truncation rank (first i with ||s[i:]|| < eps, capped; reference
+// column-major matrix X, with the rotations of each block pair computed on its
+// Gram matrix.  The columns are cut into blocks of bs <= 16; the blocks meet in
+// the round-robin tournament, one outer round per kernel triple:
+//   jac_gram   G_p = X_p^H X_p for every block pair p (2bs columns), each
+//              workgroup one chunk of 64 rows, partial Grams to a workspace;
+//   jac_inner  one workgroup per pair sums its partial Grams and runs one
+//              cyclic sweep of two-sided Jacobi rotations on G_p in LDS
+//              (G <- R^H G R), accumulating J_p = product of the rotations;
+//   jac_apply  X_p <- X_p J_p, each workgroup one chunk of rows.
+// Stream order separates the rounds.  The Gram is rebuilt from the columns in
+// every round, so the test for a converged pair, |x_i^H x_j| <=
+// tol*|x_i||x_j|, and the singular values (final column norms) carry the
+// accuracy of the one-sided method: working precision relative to the largest
+// singular value, which the 1e-10 truncation tail test needs.  A sweep without
+// rotations sets a device flag that turns the remaining launches into no-ops.
+// Columns below 1e-15 of the matrix norm are left alone (their contribution to
+// any tail is < 1e-14).  jac_finalize sorts the singular values and evaluates
+// the truncation rank (first i with ||s[i:]|| < eps, capped; reference
 // algorithms/tdvp.py:289-292).
 //
 // householder_qr: zgeqr2 + zung2r semantics (reference tensor_networks/mps.py:86
@@ -20,14 +27,17 @@
 // complete-QR complement columns.
 #include "kernels.hpp"
 
+#include <algorithm>
+
 namespace qca {
 namespace {
 
-constexpr int JAC_THREADS = 1024;
-constexpr int JAC_WAVES = JAC_THREADS / 64;
+constexpr int JAC_NT = 1024;         // threads of every Jacobi kernel
 constexpr int JAC_MAXCOLS = 512;
-constexpr int JAC_MAXR = 8;               // rows per lane (rows <= 512)
-constexpr int JAC_LDS_BYTES = 128 * 1024;  // one block pair of columns
+constexpr int JAC_BS = 16;           // block size: pairs of 32 columns
+constexpr int JAC_LMAX = 2 * JAC_BS;
+constexpr int JAC_RC = 64;           // rows per Gram / apply chunk
+constexpr int JAC_MAX_SWEEPS = 60;
 
 __device__ __forceinline__ double wsum(double v) {
 #pragma unroll
@@ -40,153 +50,226 @@ __device__ __forceinline__ int rr_player(int t, int r, int n) {
   return (t == 0) ? 0 : ((t - 1 + r) % (n - 1)) + 1;
 }
 
-// One Hestenes rotation of the column pair (xi, xj) held in LDS (rows long).
-// Returns 1 when a rotation was applied.
-__device__ int jacobi_pair(c128* xi, c128* xj, int rows, int lane, double tol, double negl) {
-  double ir[JAC_MAXR], ii[JAC_MAXR], jr[JAC_MAXR], ji[JAC_MAXR];
-  double a = 0, b = 0, gr = 0, gi = 0;
-#pragma unroll
-  for (int q = 0; q < JAC_MAXR; ++q) {
-    const int r = lane + 64 * q;
-    if (r < rows) {
-      const c128 u = xi[r], v = xj[r];
-      ir[q] = u.re; ii[q] = u.im; jr[q] = v.re; ji[q] = v.im;
-    } else {
-      ir[q] = ii[q] = jr[q] = ji[q] = 0.0;
-    }
-    a += ir[q] * ir[q] + ii[q] * ii[q];
-    b += jr[q] * jr[q] + ji[q] * ji[q];
-    gr += ir[q] * jr[q] + ii[q] * ji[q];  // g = x_i^H x_j
-    gi += ir[q] * ji[q] - ii[q] * jr[q];
-  }
-  a = wsum(a); b = wsum(b); gr = wsum(gr); gi = wsum(gi);
-  const double ag = sqrt(gr * gr + gi * gi);
-  // converged pair, or both columns below the rounding level of the matrix
-  if (ag <= tol * sqrt(a) * sqrt(b) || ag < 1e-280 || fmax(a, b) < negl) return 0;
-  const double zeta = (b - a) / (2.0 * ag);
-  double t;
-  if (fabs(zeta) > 1e100) t = 0.5 / zeta;
-  else t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-  const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
-  const double er = gr / ag, ei = gi / ag;
-  // x_i' = c x_i - s conj(e) x_j ;  x_j' = s e x_i + c x_j
-#pragma unroll
-  for (int q = 0; q < JAC_MAXR; ++q) {
-    const int r = lane + 64 * q;
-    if (r < rows) {
-      const double ce_r = er * jr[q] + ei * ji[q], ce_i = er * ji[q] - ei * jr[q];
-      const double e_r = er * ir[q] - ei * ii[q], e_i = er * ii[q] + ei * ir[q];
-      xi[r] = c128{c * ir[q] - s * ce_r, c * ii[q] - s * ce_i};
-      xj[r] = c128{s * e_r + c * jr[q], s * e_i + c * ji[q]};
-    }
-  }
-  return 1;
+struct JacobiCtrl {
+  double fro;                  // ||X||_F^2
+  int done;                    // set by the first sweep without rotations
+  int sweeps;                  // sweeps executed
+  int rot[JAC_MAX_SWEEPS];     // rotations per sweep
+};
+
+struct JacobiGeom {
+  int rows, cols, bs, nb, kspl;
+};
+
+// Global column of local column c of block pair p in round r.
+__device__ __forceinline__ int pair_col(const JacobiGeom& g, int p, int round, int c) {
+  const int bi = rr_player(p, round, g.nb), bj = rr_player(g.nb - 1 - p, round, g.nb);
+  return c < g.bs ? bi * g.bs + c : bj * g.bs + (c - g.bs);
 }
 
-// Grid barrier for a cooperative launch (all workgroups co-resident): every
-// workgroup adds 1 to bar[0] and waits until the count reaches the target of
-// this barrier instance (k-th barrier: k * gridDim.x).  Every thread fences at
-// device scope on both sides, so global stores before the barrier are visible
-// to, and stale cache lines are not read by, any thread after it.
-__device__ void grid_barrier(unsigned int* bar, unsigned int& target) {
-  target += gridDim.x;
-  __threadfence();
+// Stages rows [r0, r0+nr) of the 2bs columns of pair p into a row-major LDS
+// tile with a padded row stride (JAC_LMAX + 1) that spreads the banks.
+__device__ void stage_tile(const c128* __restrict__ X, const JacobiGeom& g, int p, int round,
+                           int r0, int nr, c128* tile) {
+  const int L = 2 * g.bs;
+  for (int e = threadIdx.x; e < L * nr; e += JAC_NT) {
+    const int c = e / nr, r = e - c * nr;
+    const int gc = pair_col(g, p, round, c);
+    tile[r * (JAC_LMAX + 1) + c] = gc < g.cols ? X[(long)gc * g.rows + r0 + r] : c128{0.0, 0.0};
+  }
+}
+
+// ||X||_F^2 and reset of the control block (one workgroup).
+__global__ __launch_bounds__(JAC_NT) void jac_init(const c128* __restrict__ X, long n,
+                                                   JacobiCtrl* __restrict__ ctrl) {
+  __shared__ double red[JAC_NT / 64];
+  double part = 0;
+  for (long e = threadIdx.x; e < n; e += JAC_NT) part += X[e].re * X[e].re + X[e].im * X[e].im;
+  part = wsum(part);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
   __syncthreads();
   if (threadIdx.x == 0) {
-    atomicAdd(bar, 1u);
-    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      __builtin_amdgcn_s_sleep(1);
-    }
+    double s = 0;
+    for (int w = 0; w < JAC_NT / 64; ++w) s += red[w];
+    ctrl->fro = s;
+    ctrl->done = 0;
+    ctrl->sweeps = 0;
   }
-  __syncthreads();
-  __threadfence();
+  if (threadIdx.x < JAC_MAX_SWEEPS) ctrl->rot[threadIdx.x] = 0;
 }
 
-__global__ void zero_words(unsigned int* p, int n) {
-  if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+// Partial Gram of pair blockIdx.x over row chunk blockIdx.y:
+// Gpart[p][kc](i, j) = sum_r conj(X(r, c_i)) X(r, c_j), stored column-major.
+__global__ __launch_bounds__(JAC_NT) void jac_gram(const c128* __restrict__ X, JacobiGeom g,
+                                                   int round, c128* __restrict__ Gpart,
+                                                   const JacobiCtrl* __restrict__ ctrl) {
+  if (ctrl->done) return;
+  __shared__ c128 tile[JAC_RC * (JAC_LMAX + 1)];
+  const int p = blockIdx.x, kc = blockIdx.y, L = 2 * g.bs;
+  const int r0 = kc * JAC_RC, nr = min(JAC_RC, g.rows - r0);
+  stage_tile(X, g, p, round, r0, nr, tile);
+  __syncthreads();
+  c128* out = Gpart + ((long)p * g.kspl + kc) * L * L;
+  for (int e = threadIdx.x; e < L * L; e += JAC_NT) {
+    const int i = e % L, j = e / L;
+    double re = 0, im = 0;
+    for (int r = 0; r < nr; ++r) {
+      const c128 u = tile[r * (JAC_LMAX + 1) + i], v = tile[r * (JAC_LMAX + 1) + j];
+      re += u.re * v.re + u.im * v.im;
+      im += u.re * v.im - u.im * v.re;
+    }
+    out[e] = c128{re, im};
+  }
 }
 
-// Block one-sided Jacobi.  gridDim.x = nb/2 workgroups, each sweeping one pair
-// of column blocks (bs columns each) in LDS per round; rounds follow the
-// round-robin tournament over the nb blocks, separated by grid barriers.  The
-// loop ends after a sweep without rotations (all workgroups read the same
-// counter after a barrier) or after max_sweeps.  Workgroup 0 then sorts the
-// column norms and evaluates the truncation rank.
-__global__ __launch_bounds__(JAC_THREADS) void block_jacobi_kernel(
-    c128* __restrict__ X, int rows, int cols, int bs, int nb, int max_sweeps, double tol,
-    unsigned int* __restrict__ bar, int* __restrict__ rotc, double* __restrict__ fro,
-    double* __restrict__ sigma, int* __restrict__ perm, SvdResult* __restrict__ res, int chi_max,
-    double eps, int rank_cap) {
-  extern __shared__ unsigned char jac_smem[];
-  c128* lds = reinterpret_cast<c128*>(jac_smem);  // [2 bs][rows]
-  __shared__ int local_rot;
-  __shared__ double red[JAC_WAVES];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wg = blockIdx.x, nwg = gridDim.x;
-  unsigned int target = 0;
-
-  // Frobenius norm^2 of X (for the negligible-column threshold)
-  double part = 0;
-  for (long e = (long)wg * JAC_THREADS + tid; e < (long)rows * cols; e += (long)nwg * JAC_THREADS)
-    part += X[e].re * X[e].re + X[e].im * X[e].im;
-  part = wsum(part);
-  if (lane == 0) red[wave] = part;
+// One cyclic sweep of two-sided Jacobi on the Gram of pair blockIdx.x.
+// Rotation of columns (i, j), with g = G(i, j) = x_i^H x_j, e = g/|g|:
+//   x_i' = c x_i - s conj(e) x_j,   x_j' = s e x_i + c x_j,
+// t = sign(zeta)/(|zeta| + sqrt(1 + zeta^2)), zeta = (|x_j|^2 - |x_i|^2)/(2|g|),
+// c = 1/sqrt(1 + t^2), s = t c.  G <- R^H G R and J <- J R.
+__global__ __launch_bounds__(JAC_NT) void jac_inner(JacobiGeom g, int sweep,
+                                                    const c128* __restrict__ Gpart,
+                                                    c128* __restrict__ Jout,
+                                                    int* __restrict__ rotated, double tol,
+                                                    JacobiCtrl* __restrict__ ctrl) {
+  if (ctrl->done) return;
+  __shared__ c128 G[JAC_LMAX * JAC_LMAX];
+  __shared__ c128 J[JAC_LMAX * JAC_LMAX];
+  __shared__ double rc[JAC_LMAX / 2], rsr[JAC_LMAX / 2], rsi[JAC_LMAX / 2];
+  __shared__ int ract[JAC_LMAX / 2];
+  __shared__ int total;
+  const int p = blockIdx.x, L = 2 * g.bs, H = g.bs, tid = threadIdx.x;
+  const double negl = 1e-30 * ctrl->fro;
+  const double tol2 = tol * tol;
+  for (int e = tid; e < L * L; e += JAC_NT) {
+    double re = 0, im = 0;
+    for (int kc = 0; kc < g.kspl; ++kc) {
+      const c128 v = Gpart[((long)p * g.kspl + kc) * L * L + e];
+      re += v.re;
+      im += v.im;
+    }
+    G[e] = c128{re, im};
+    J[e] = c128{(e % L) == (e / L) ? 1.0 : 0.0, 0.0};
+  }
+  if (tid == 0) total = 0;
+  int mine = 0;
   __syncthreads();
+  for (int round = 0; round < L - 1; ++round) {
+    if (tid < H) {
+      const int i = rr_player(tid, round, L), j = rr_player(L - 1 - tid, round, L);
+      const double a = G[i * L + i].re, b = G[j * L + j].re;
+      const c128 gij = G[j * L + i];
+      const double ag2 = gij.re * gij.re + gij.im * gij.im;
+      int act = !(ag2 <= tol2 * a * b || ag2 < 1e-300 || fmax(a, b) < negl);
+      double c = 1.0, sr = 0.0, si = 0.0;
+      if (act) {
+        // correctly rounded sqrt and division keep c^2 + s^2 and |e| within
+        // unbiased rounding of 1, so norm errors do not drift over the sweeps
+        const double ag = sqrt(ag2);
+        const double zeta = (b - a) / (2.0 * ag);
+        double t;
+        if (fabs(zeta) > 1e100) t = 0.5 / zeta;
+        else t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        c = 1.0 / sqrt(1.0 + t * t);
+        const double sv = t * c;
+        sr = sv * (gij.re / ag);  // s e
+        si = sv * (gij.im / ag);
+        ++mine;
+      }
+      rc[tid] = c; rsr[tid] = sr; rsi[tid] = si; ract[tid] = act;
+    }
+    __syncthreads();
+    // items [0, H*H): 2x2 blocks (rows of pair P, columns of pair Q) of G;
+    // items [H*H, H*H + L*H): row r of J, pair Q
+    for (int item = tid; item < H * H + L * H; item += JAC_NT) {
+      if (item < H * H) {
+        const int P = item % H, Q = item / H;
+        if (!ract[P] && !ract[Q]) continue;
+        const int pi = rr_player(P, round, L), pj = rr_player(L - 1 - P, round, L);
+        const int qi = rr_player(Q, round, L), qj = rr_player(L - 1 - Q, round, L);
+        const double pc = rc[P], psr = rsr[P], psi = rsi[P];
+        const double qc = rc[Q], qsr = rsr[Q], qsi = rsi[Q];
+        const c128 t00 = G[qi * L + pi], t10 = G[qi * L + pj], t01 = G[qj * L + pi], t11 = G[qj * L + pj];
+        // right: col_i' = c col_i - conj(se) col_j ; col_j' = se col_i + c col_j
+        const c128 u00{qc * t00.re - (qsr * t01.re + qsi * t01.im), qc * t00.im - (qsr * t01.im - qsi * t01.re)};
+        const c128 u10{qc * t10.re - (qsr * t11.re + qsi * t11.im), qc * t10.im - (qsr * t11.im - qsi * t11.re)};
+        const c128 u01{(qsr * t00.re - qsi * t00.im) + qc * t01.re, (qsr * t00.im + qsi * t00.re) + qc * t01.im};
+        const c128 u11{(qsr * t10.re - qsi * t10.im) + qc * t11.re, (qsr * t10.im + qsi * t10.re) + qc * t11.im};
+        // left by R^H: row_i' = c row_i - se row_j ; row_j' = conj(se) row_i + c row_j
+        G[qi * L + pi] = c128{pc * u00.re - (psr * u10.re - psi * u10.im), pc * u00.im - (psr * u10.im + psi * u10.re)};
+        G[qi * L + pj] = c128{(psr * u00.re + psi * u00.im) + pc * u10.re, (psr * u00.im - psi * u00.re) + pc * u10.im};
+        G[qj * L + pi] = c128{pc * u01.re - (psr * u11.re - psi * u11.im), pc * u01.im - (psr * u11.im + psi * u11.re)};
+        G[qj * L + pj] = c128{(psr * u01.re + psi * u01.im) + pc * u11.re, (psr * u01.im - psi * u01.re) + pc * u11.im};
+      } else {
+        const int it = item - H * H, r = it % L, Q = it / L;
+        if (!ract[Q]) continue;
+        const int qi = rr_player(Q, round, L), qj = rr_player(L - 1 - Q, round, L);
+        const double qc = rc[Q], qsr = rsr[Q], qsi = rsi[Q];
+        const c128 x = J[qi * L + r], y = J[qj * L + r];
+        J[qi * L + r] = c128{qc * x.re - (qsr * y.re + qsi * y.im), qc * x.im - (qsr * y.im - qsi * y.re)};
+        J[qj * L + r] = c128{(qsr * x.re - qsi * x.im) + qc * y.re, (qsr * x.im + qsi * x.re) + qc * y.im};
+      }
+    }
+    __syncthreads();
+  }
+  if (mine) atomicAdd(&total, mine);
+  __syncthreads();
+  c128* jo = Jout + (long)p * L * L;
+  for (int e = tid; e < L * L; e += JAC_NT) jo[e] = J[e];
   if (tid == 0) {
-    double sacc = 0;
-    for (int w = 0; w < JAC_WAVES; ++w) sacc += red[w];
-    atomicAdd(fro, sacc);
+    rotated[p] = total;
+    if (total) atomicAdd(&ctrl->rot[sweep], total);
   }
-  grid_barrier(bar, target);
-  const double negl = 1e-30 * __hip_atomic_load(fro, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
-  const int lc = 2 * bs;  // local columns
-  int sweep = 0, converged = 0;
-  for (sweep = 0; sweep < max_sweeps; ++sweep) {
-    if (tid == 0) local_rot = 0;
-    for (int round = 0; round < nb - 1; ++round) {
-      const int bi = rr_player(wg, round, nb), bj = rr_player(nb - 1 - wg, round, nb);
-      // stage the two blocks (zero columns beyond cols)
-      for (int e = tid; e < lc * rows; e += JAC_THREADS) {
-        const int c = e / rows, r = e - c * rows;
-        const int gc = (c < bs ? bi * bs + c : bj * bs + (c - bs));
-        lds[e] = (gc < cols) ? X[(long)gc * rows + r] : c128{0.0, 0.0};
-      }
-      __syncthreads();
-      // one inner sweep over the 2 bs local columns
-      for (int ir = 0; ir < lc - 1; ++ir) {
-        for (int pr = wave; pr < bs; pr += JAC_WAVES) {
-          const int ci = rr_player(pr, ir, lc), cj = rr_player(lc - 1 - pr, ir, lc);
-          const int n = jacobi_pair(lds + (long)ci * rows, lds + (long)cj * rows, rows, lane, tol, negl);
-          if (n && lane == 0) atomicAdd(&local_rot, 1);
-        }
-        __syncthreads();
-      }
-      for (int e = tid; e < lc * rows; e += JAC_THREADS) {
-        const int c = e / rows, r = e - c * rows;
-        const int gc = (c < bs ? bi * bs + c : bj * bs + (c - bs));
-        if (gc < cols) X[(long)gc * rows + r] = lds[e];
-      }
-      grid_barrier(bar, target);
+// X_p <- X_p J_p on row chunk blockIdx.y of pair blockIdx.x.
+__global__ __launch_bounds__(JAC_NT) void jac_apply(c128* __restrict__ X, JacobiGeom g, int round,
+                                                    const c128* __restrict__ Jin,
+                                                    const int* __restrict__ rotated,
+                                                    const JacobiCtrl* __restrict__ ctrl) {
+  if (ctrl->done || rotated[blockIdx.x] == 0) return;
+  __shared__ c128 tile[JAC_RC * (JAC_LMAX + 1)];
+  __shared__ c128 J[JAC_LMAX * JAC_LMAX];
+  const int p = blockIdx.x, kc = blockIdx.y, L = 2 * g.bs;
+  const int r0 = kc * JAC_RC, nr = min(JAC_RC, g.rows - r0);
+  stage_tile(X, g, p, round, r0, nr, tile);
+  const c128* jp = Jin + (long)p * L * L;
+  for (int e = threadIdx.x; e < L * L; e += JAC_NT) J[e] = jp[e];
+  __syncthreads();
+  for (int e = threadIdx.x; e < L * nr; e += JAC_NT) {
+    const int c = e / nr, r = e - c * nr;
+    const int gc = pair_col(g, p, round, c);
+    if (gc >= g.cols) continue;
+    double re = 0, im = 0;
+    for (int k = 0; k < L; ++k) {
+      const c128 x = tile[r * (JAC_LMAX + 1) + k], j = J[c * L + k];
+      re += x.re * j.re - x.im * j.im;
+      im += x.re * j.im + x.im * j.re;
     }
-    if (tid == 0) atomicAdd(&rotc[sweep & 1], local_rot);
-    grid_barrier(bar, target);
-    const int total = __hip_atomic_load(&rotc[sweep & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (wg == 0 && tid == 0) rotc[(sweep + 1) & 1] = 0;
-    grid_barrier(bar, target);
-    if (total == 0) {
-      converged = 1;
-      break;
-    }
+    X[(long)gc * g.rows + r0 + r] = c128{re, im};
   }
-  if (wg != 0) return;
+}
 
-  // ---- workgroup 0: column norms, descending sort, truncation rank
-  double* skey = reinterpret_cast<double*>(jac_smem);            // reuse LDS
-  int* sidx = reinterpret_cast<int*>(skey + JAC_MAXCOLS);
+// Closes a sweep: converged when it made no rotation.
+__global__ void jac_check(JacobiCtrl* __restrict__ ctrl, int sweep) {
+  if (threadIdx.x != 0 || ctrl->done) return;
+  ctrl->sweeps = sweep + 1;
+  if (ctrl->rot[sweep] == 0) ctrl->done = 1;
+}
+
+// Column norms, descending sort and truncation rank (one workgroup).
+__global__ __launch_bounds__(JAC_NT) void jac_finalize(const c128* __restrict__ X, int rows, int cols,
+                                                       double* __restrict__ sigma,
+                                                       int* __restrict__ perm,
+                                                       SvdResult* __restrict__ res, int chi_max,
+                                                       double eps, int rank_cap,
+                                                       const JacobiCtrl* __restrict__ ctrl) {
+  __shared__ double skey[JAC_MAXCOLS];
+  __shared__ int sidx[JAC_MAXCOLS];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = JAC_NT / 64;
   int np2 = 1;
   while (np2 < cols) np2 <<= 1;
-  for (int c = wave; c < np2; c += JAC_WAVES) {
+  for (int c = wave; c < np2; c += nw) {
     double a = 0;
     if (c < cols)
       for (int r = lane; r < rows; r += 64) {
@@ -202,23 +285,23 @@ __global__ __launch_bounds__(JAC_THREADS) void block_jacobi_kernel(
   __syncthreads();
   for (int size = 2; size <= np2; size <<= 1) {
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int e = tid; e < np2; e += JAC_THREADS) {
-        const int p = e ^ stride;
-        if (p > e) {
+      for (int e = tid; e < np2; e += JAC_NT) {
+        const int q = e ^ stride;
+        if (q > e) {
           const bool desc = ((e & size) == 0);
-          const double ke = skey[e], kp = skey[p];
-          const bool swap = desc ? (ke < kp || (ke == kp && sidx[e] > sidx[p]))
-                                 : (ke > kp || (ke == kp && sidx[e] < sidx[p]));
+          const double ke = skey[e], kq = skey[q];
+          const bool swap = desc ? (ke < kq || (ke == kq && sidx[e] > sidx[q]))
+                                 : (ke > kq || (ke == kq && sidx[e] < sidx[q]));
           if (swap) {
-            skey[e] = kp; skey[p] = ke;
-            const int ti = sidx[e]; sidx[e] = sidx[p]; sidx[p] = ti;
+            skey[e] = kq; skey[q] = ke;
+            const int ti = sidx[e]; sidx[e] = sidx[q]; sidx[q] = ti;
           }
         }
       }
       __syncthreads();
     }
   }
-  for (int e = tid; e < cols; e += JAC_THREADS) {
+  for (int e = tid; e < cols; e += JAC_NT) {
     sigma[e] = skey[e];
     perm[e] = sidx[e];
   }
@@ -238,8 +321,8 @@ __global__ __launch_bounds__(JAC_THREADS) void block_jacobi_kernel(
     for (int e = 0; e < k; ++e) kept += skey[e] * skey[e];
     for (int e = k; e < ns; ++e) tail += skey[e] * skey[e];
     res->k = k;
-    res->nsweeps = sweep + (converged ? 1 : 0);
-    res->converged = converged;
+    res->nsweeps = ctrl->sweeps;
+    res->converged = ctrl->done;
     res->norm_kept = sqrt(kept);
     res->tail = sqrt(tail);
   }
@@ -362,47 +445,71 @@ __global__ __launch_bounds__(QR_THREADS) void householder_qr_kernel(c128* __rest
 
 }  // namespace
 
+JacobiLayout jacobi_layout(int rows, int cols) {
+  JacobiLayout l;
+  l.bs = cols <= JAC_LMAX ? std::max(1, (cols + 1) / 2) : JAC_BS;
+  l.nb = std::max(2, ceil_div(cols, l.bs));
+  if (l.nb & 1) ++l.nb;
+  l.kspl = ceil_div(rows, JAC_RC);
+  const long L = 2L * l.bs, np = l.nb / 2;
+  l.gpart_elems = np * l.kspl * L * L;
+  l.j_elems = np * L * L;
+  l.bytes = 256 + sizeof(int) * np + sizeof(c128) * (l.gpart_elems + l.j_elems);
+  return l;
+}
+
+size_t jacobi_workspace_bytes(int max_rows, int max_cols) {
+  return jacobi_layout(max_rows, max_cols).bytes + 1024;
+}
+
 void launch_jacobi_svd(c128* X, int rows, int cols, double* sigma, int* perm, SvdResult* res,
                        int chi_max, double eps, int rank_cap, hipStream_t s, JacobiWork& w) {
-  QCA_CHECK(rows >= 1 && rows <= 64 * JAC_MAXR, "SVD rows out of range (1..512)");
+  QCA_CHECK(rows >= 1 && rows <= 512, "SVD rows out of range (1..512)");
   QCA_CHECK(cols >= 1 && cols <= JAC_MAXCOLS, "SVD cols out of range (1..512)");
-  // block size: a pair of blocks (2 bs columns of `rows` complex doubles) fits
-  // in 128 KiB of LDS, and is no wider than the matrix needs
-  int bs = 1;
-  while (bs * 2 <= 64 && 2L * (bs * 2) * rows * (long)sizeof(c128) <= JAC_LDS_BYTES &&
-         2 * bs < cols)
-    bs *= 2;
-  int nb = ceil_div(cols, bs);
-  if (nb < 2) nb = 2;
-  if (nb & 1) ++nb;
-  const int nwg = nb / 2;
-  const size_t shm = std::max<size_t>(2UL * bs * rows * sizeof(c128),
-                                      JAC_MAXCOLS * (sizeof(double) + sizeof(int)));
-  static size_t attr_bytes = 0;  // dynamic LDS opt-in (static + dynamic <= 160 KiB)
-  if (shm > attr_bytes) {
-    QCA_HIP(hipFuncSetAttribute((const void*)block_jacobi_kernel,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    attr_bytes = shm;
-  }
-  hipLaunchKernelGGL(zero_words, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned int*>(w.sync), 16);
-  int max_sweeps = 60;
-  double tol = 1.1102230246251565e-16 * sqrt((double)rows);
-  unsigned int* bar = reinterpret_cast<unsigned int*>(w.sync);
-  int* rotc = reinterpret_cast<int*>(w.sync) + 4;
-  double* fro = reinterpret_cast<double*>(reinterpret_cast<char*>(w.sync) + 32);
-  void* args[] = {&X, &rows, &cols, &bs, &nb, &max_sweeps, &tol, &bar, &rotc, &fro,
-                  &sigma, &perm, &res, &chi_max, &eps, &rank_cap};
-  if (nwg == 1) {
-    // one workgroup: the grid barrier reduces to a workgroup barrier
-    hipLaunchKernelGGL(block_jacobi_kernel, dim3(1), dim3(JAC_THREADS), shm, s, X, rows, cols, bs, nb,
-                       max_sweeps, tol, bar, rotc, fro, sigma, perm, res, chi_max, eps, rank_cap);
+  QCA_CHECK(w.ws != nullptr && w.host != nullptr, "Jacobi workspace not set");
+  const JacobiLayout l = jacobi_layout(rows, cols);
+  QCA_CHECK(l.bytes <= w.ws_bytes, "Jacobi workspace too small");
+  static_assert(sizeof(JacobiCtrl) <= 256, "control block exceeds its slot");
+  unsigned char* base = reinterpret_cast<unsigned char*>(w.ws);
+  JacobiCtrl* ctrl = reinterpret_cast<JacobiCtrl*>(base);
+  int* rotated = reinterpret_cast<int*>(base + 256);
+  c128* Gpart = reinterpret_cast<c128*>(base + 256 + ((sizeof(int) * (l.nb / 2) + 15) / 16) * 16);
+  c128* Jbuf = Gpart + l.gpart_elems;
+  QCA_CHECK((reinterpret_cast<unsigned char*>(Jbuf + l.j_elems) - base) <= (long)w.ws_bytes,
+            "Jacobi workspace too small");
+  const JacobiGeom g{rows, cols, l.bs, l.nb, l.kspl};
+  const int np = l.nb / 2;
+  const double tol = 1.1102230246251565e-16 * sqrt((double)rows);
+  hipLaunchKernelGGL(jac_init, dim3(1), dim3(JAC_NT), 0, s, X, (long)rows * cols, ctrl);
+  // launch the sweeps predicted from the previous call, then read the device
+  // flag once per chunk (rounds after convergence are no-ops)
+  int sweep = 0;
+  int chunk = std::max(2, w.pred_sweeps);
+  while (sweep < JAC_MAX_SWEEPS) {
+    for (int c = 0; c < chunk && sweep < JAC_MAX_SWEEPS; ++c, ++sweep) {
+      for (int r = 0; r < l.nb - 1; ++r) {
+        hipLaunchKernelGGL(jac_gram, dim3(np, l.kspl), dim3(JAC_NT), 0, s, (const c128*)X, g, r,
+                           Gpart, (const JacobiCtrl*)ctrl);
+        hipLaunchKernelGGL(jac_inner, dim3(np), dim3(JAC_NT), 0, s, g, sweep, (const c128*)Gpart,
+                           Jbuf, rotated, tol, ctrl);
+        hipLaunchKernelGGL(jac_apply, dim3(np, l.kspl), dim3(JAC_NT), 0, s, X, g, r,
+                           (const c128*)Jbuf, (const int*)rotated, (const JacobiCtrl*)ctrl);
+      }
+      hipLaunchKernelGGL(jac_check, dim3(1), dim3(64), 0, s, ctrl, sweep);
+    }
     QCA_HIP(hipGetLastError());
-  } else {
-    // several workgroups wait on each other: cooperative launch guarantees
-    // they are co-resident (or fails instead of hanging)
-    QCA_HIP(hipLaunchCooperativeKernel((const void*)block_jacobi_kernel, dim3(nwg),
-                                       dim3(JAC_THREADS), args, (unsigned int)shm, s));
+    QCA_HIP(hipMemcpyAsync(w.host, &ctrl->done, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    QCA_HIP(hipStreamSynchronize(s));
+    const int* hc = reinterpret_cast<const int*>(w.host);
+    if (hc[0]) {
+      w.pred_sweeps = hc[1];
+      break;
+    }
+    chunk = 2;
   }
+  hipLaunchKernelGGL(jac_finalize, dim3(1), dim3(JAC_NT), 0, s, (const c128*)X, rows, cols, sigma,
+                     perm, res, chi_max, eps, rank_cap, (const JacobiCtrl*)ctrl);
+  QCA_HIP(hipGetLastError());
 }
 
 void launch_householder_qr(c128* M, int rows, int cols, c128* tau, c128* Q, int qcols,

@@ -32,7 +32,8 @@ class EngineParams(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("matvecs", c_long), ("matvec_2site", c_long), ("matvec_1site", c_long),
                 ("matvec_bond", c_long), ("site_updates", c_long), ("svd_sweeps", c_long),
-                ("lanczos_limit_hits", c_long), ("flops", c_double)]
+                ("lanczos_limit_hits", c_long), ("svd_calls", c_long), ("flops", c_double),
+                ("svd_ms", c_double), ("krylov_ms", c_double)]
 
 
 # name -> (restype, argtypes); mirrors include/qca.h
