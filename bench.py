@@ -97,6 +97,21 @@ def heff2_roofline(torch, prec: int, reps: int = 50):
     return ms, flops
 
 
+def profiler_control(enabled: bool):
+    """Returns on(bool): resume/pause rocprofv3 collection (roctx profiler
+    control) when `enabled`, a no-op otherwise.  Collection starts paused."""
+    if not enabled:
+        return lambda on: None
+    path = "/opt/rocm/lib/librocprofiler-sdk-roctx.so"
+    lib = ctypes.CDLL(path if os.path.exists(path) else "librocprofiler-sdk-roctx.so")
+    lib.roctxProfilerPause.argtypes = [ctypes.c_int]
+    lib.roctxProfilerResume.argtypes = [ctypes.c_int]
+
+    def on(resume: bool) -> None:
+        (lib.roctxProfilerResume if resume else lib.roctxProfilerPause)(0)
+    return on
+
+
 def cpu_baseline(As, seconds: float):
     """Oracle 2TDVP site-updates on the host at the benchmark state (bounded)."""
     from oracle import automaton as AU, tdvp as OT
@@ -132,6 +147,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-timed", action="store_true",
+                    help="pause rocprofv3 collection outside the timed steps (run under "
+                         "rocprofv3 --selected-regions)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the H_eff micro-benchmark and print its roofline object "
                          "(the command the kernel profiles are taken with)")
@@ -168,6 +186,7 @@ def main() -> int:
     engines = {"mixed": qn.Engine(Ws, engines["fp64"].tensors(), algorithm=2, precision=1, chi_max=CHI,
                                   dt=DT, eps=EPS, krylov_tol=1e-12),
                "fp64": engines["fp64"]}
+    prof = profiler_control(a.profile_timed)
     maxdiff = 0.0
     times = {"mixed": 0.0, "fp64": 0.0}
     gpu_ms = {"mixed": 0.0, "fp64": 0.0}
@@ -177,10 +196,12 @@ def main() -> int:
     for k in range(a.steps):
         for name, e in engines.items():
             torch.cuda.synchronize()
+            prof(True)
             t0 = time.perf_counter()
             e.step(1)
             torch.cuda.synchronize()
             times[name] += time.perf_counter() - t0
+            prof(False)
         d = np.abs(engines["mixed"].rho()[:, 1, 1].real - engines["fp64"].rho()[:, 1, 1].real).max()
         maxdiff = max(maxdiff, d)
         log(f"timed step {k + 1}/{a.steps}: mixed {times['mixed']:.2f}s fp64 {times['fp64']:.2f}s |dn| {d:.2e}")

@@ -123,20 +123,43 @@ __global__ __launch_bounds__(JAC_NT) void jac_gram(const c128* __restrict__ X, J
 }
 
 // One cyclic sweep of two-sided Jacobi on the Gram of pair blockIdx.x.
-// Rotation of columns (i, j), with g = G(i, j) = x_i^H x_j, e = g/|g|:
-//   x_i' = c x_i - s conj(e) x_j,   x_j' = s e x_i + c x_j,
-// t = sign(zeta)/(|zeta| + sqrt(1 + zeta^2)), zeta = (|x_j|^2 - |x_i|^2)/(2|g|),
-// c = 1/sqrt(1 + t^2), s = t c.  G <- R^H G R and J <- J R.
+// Rotation of columns (i, j), with g = G(i, j) = x_i^H x_j, d = |x_j|^2 - |x_i|^2,
+// h = sqrt(d^2 + 4|g|^2), w = h + |d|:
+//   x_i' = c x_i - conj(s e) x_j,   x_j' = (s e) x_i + c x_j,
+//   c = sqrt(w / (2h)),  s e = sign(d) g sqrt(2 / (h w)),
+// the Rutishauser rotation t = sign(d)/(|zeta| + sqrt(1 + zeta^2)), zeta = d/(2|g|),
+// c = 1/sqrt(1 + t^2), s = t c, rewritten so that c^2 + |s e|^2 = 1 exactly in
+// real arithmetic, every operation is a correctly rounded add, multiply,
+// divide or square root on non-negative operands, and the critical path is
+// sqrt -> divide -> sqrt.  G <- R^H G R and J <- J R.
+// Per round: wave 0 computes the rotations of round r while the other waves
+// apply round r-1 to J; after a barrier all threads apply round r to G.
+// The pair indices are recomputed in registers, the rotation of a pair is one
+// 32-byte LDS record.
+struct alignas(16) JRot {
+  double c, sr, si;   // c, s e
+  int act, pad;
+};
+static_assert(JAC_NT >= 64 + JAC_LMAX * (JAC_LMAX / 2), "jac_inner needs one thread per J item");
+static_assert(JAC_NT >= (JAC_LMAX / 2) * (JAC_LMAX / 2), "jac_inner needs one thread per G block");
+
+// Round-robin position rr_player(t, r, n) without integer division (t < n, r < n - 1).
+__device__ __forceinline__ int rr_fast(int t, int r, int n) {
+  if (t == 0) return 0;
+  int x = t - 1 + r;
+  x -= (x >= n - 1) ? (n - 1) : 0;
+  return x + 1;
+}
 __global__ __launch_bounds__(JAC_NT) void jac_inner(JacobiGeom g, int sweep,
                                                     const c128* __restrict__ Gpart,
                                                     c128* __restrict__ Jout,
                                                     int* __restrict__ rotated, double tol,
                                                     JacobiCtrl* __restrict__ ctrl) {
   if (ctrl->done) return;
+  constexpr int HM = JAC_LMAX / 2;
   __shared__ c128 G[JAC_LMAX * JAC_LMAX];
   __shared__ c128 J[JAC_LMAX * JAC_LMAX];
-  __shared__ double rc[JAC_LMAX / 2], rsr[JAC_LMAX / 2], rsi[JAC_LMAX / 2];
-  __shared__ int ract[JAC_LMAX / 2];
+  __shared__ JRot rot[2][HM];
   __shared__ int total;
   const int p = blockIdx.x, L = 2 * g.bs, H = g.bs, tid = threadIdx.x;
   const double negl = 1e-30 * ctrl->fro;
@@ -153,61 +176,61 @@ __global__ __launch_bounds__(JAC_NT) void jac_inner(JacobiGeom g, int sweep,
   }
   if (tid == 0) total = 0;
   int mine = 0;
+  // fixed roles: lanes < H of wave 0 compute the rotations; threads 64.. own
+  // one J item (row r, pair Q) each; threads < H*H own one 2x2 block of G
+  const int jit = tid - 64;
+  const bool has_j = jit >= 0 && jit < L * H;
+  const int jr = has_j ? jit % L : 0, jq = has_j ? jit / L : 0;
+  const bool has_g = tid < H * H;
+  const int gP = tid % H, gQ = tid / H;
   __syncthreads();
-  for (int round = 0; round < L - 1; ++round) {
-    if (tid < H) {
-      const int i = rr_player(tid, round, L), j = rr_player(L - 1 - tid, round, L);
-      const double a = G[i * L + i].re, b = G[j * L + j].re;
-      const c128 gij = G[j * L + i];
-      const double ag2 = gij.re * gij.re + gij.im * gij.im;
-      int act = !(ag2 <= tol2 * a * b || ag2 < 1e-300 || fmax(a, b) < negl);
-      double c = 1.0, sr = 0.0, si = 0.0;
-      if (act) {
-        // correctly rounded sqrt and division keep c^2 + s^2 and |e| within
-        // unbiased rounding of 1, so norm errors do not drift over the sweeps
-        const double ag = sqrt(ag2);
-        const double zeta = (b - a) / (2.0 * ag);
-        double t;
-        if (fabs(zeta) > 1e100) t = 0.5 / zeta;
-        else t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-        c = 1.0 / sqrt(1.0 + t * t);
-        const double sv = t * c;
-        sr = sv * (gij.re / ag);  // s e
-        si = sv * (gij.im / ag);
-        ++mine;
+  for (int round = 0; round < L; ++round) {
+    const int cur = round & 1, prev = cur ^ 1;
+    if (tid < 64) {
+      if (round < L - 1 && tid < H) {
+        const int i = rr_fast(tid, round, L), j = rr_fast(L - 1 - tid, round, L);
+        const double a = G[i * L + i].re, b = G[j * L + j].re;
+        const c128 gij = G[j * L + i];
+        const double ag2 = gij.re * gij.re + gij.im * gij.im;
+        const int act = !(ag2 <= tol2 * a * b || ag2 < 1e-300 || fmax(a, b) < negl);
+        JRot q{1.0, 0.0, 0.0, act, 0};
+        if (act) {
+          const double d = b - a;
+          const double h = sqrt(d * d + 4.0 * ag2);
+          const double w = h + fabs(d);
+          q.c = sqrt(w / (2.0 * h));
+          const double s = (d >= 0 ? 1.0 : -1.0) * sqrt(2.0 / (h * w));
+          q.sr = s * gij.re;
+          q.si = s * gij.im;
+          ++mine;
+        }
+        rot[cur][tid] = q;
       }
-      rc[tid] = c; rsr[tid] = sr; rsi[tid] = si; ract[tid] = act;
+    } else if (round > 0 && has_j) {
+      const int qi = rr_fast(jq, round - 1, L), qj = rr_fast(L - 1 - jq, round - 1, L);
+      const c128 x = J[qi * L + jr], y = J[qj * L + jr];
+      const JRot Q = rot[prev][jq];
+      if (Q.act) {
+        J[qi * L + jr] = c128{Q.c * x.re - (Q.sr * y.re + Q.si * y.im), Q.c * x.im - (Q.sr * y.im - Q.si * y.re)};
+        J[qj * L + jr] = c128{(Q.sr * x.re - Q.si * x.im) + Q.c * y.re, (Q.sr * x.im + Q.si * x.re) + Q.c * y.im};
+      }
     }
     __syncthreads();
-    // items [0, H*H): 2x2 blocks (rows of pair P, columns of pair Q) of G;
-    // items [H*H, H*H + L*H): row r of J, pair Q
-    for (int item = tid; item < H * H + L * H; item += JAC_NT) {
-      if (item < H * H) {
-        const int P = item % H, Q = item / H;
-        if (!ract[P] && !ract[Q]) continue;
-        const int pi = rr_player(P, round, L), pj = rr_player(L - 1 - P, round, L);
-        const int qi = rr_player(Q, round, L), qj = rr_player(L - 1 - Q, round, L);
-        const double pc = rc[P], psr = rsr[P], psi = rsi[P];
-        const double qc = rc[Q], qsr = rsr[Q], qsi = rsi[Q];
-        const c128 t00 = G[qi * L + pi], t10 = G[qi * L + pj], t01 = G[qj * L + pi], t11 = G[qj * L + pj];
-        // right: col_i' = c col_i - conj(se) col_j ; col_j' = se col_i + c col_j
-        const c128 u00{qc * t00.re - (qsr * t01.re + qsi * t01.im), qc * t00.im - (qsr * t01.im - qsi * t01.re)};
-        const c128 u10{qc * t10.re - (qsr * t11.re + qsi * t11.im), qc * t10.im - (qsr * t11.im - qsi * t11.re)};
-        const c128 u01{(qsr * t00.re - qsi * t00.im) + qc * t01.re, (qsr * t00.im + qsi * t00.re) + qc * t01.im};
-        const c128 u11{(qsr * t10.re - qsi * t10.im) + qc * t11.re, (qsr * t10.im + qsi * t10.re) + qc * t11.im};
-        // left by R^H: row_i' = c row_i - se row_j ; row_j' = conj(se) row_i + c row_j
-        G[qi * L + pi] = c128{pc * u00.re - (psr * u10.re - psi * u10.im), pc * u00.im - (psr * u10.im + psi * u10.re)};
-        G[qi * L + pj] = c128{(psr * u00.re + psi * u00.im) + pc * u10.re, (psr * u00.im - psi * u00.re) + pc * u10.im};
-        G[qj * L + pi] = c128{pc * u01.re - (psr * u11.re - psi * u11.im), pc * u01.im - (psr * u11.im + psi * u11.re)};
-        G[qj * L + pj] = c128{(psr * u01.re + psi * u01.im) + pc * u11.re, (psr * u01.im - psi * u01.re) + pc * u11.im};
-      } else {
-        const int it = item - H * H, r = it % L, Q = it / L;
-        if (!ract[Q]) continue;
-        const int qi = rr_player(Q, round, L), qj = rr_player(L - 1 - Q, round, L);
-        const double qc = rc[Q], qsr = rsr[Q], qsi = rsi[Q];
-        const c128 x = J[qi * L + r], y = J[qj * L + r];
-        J[qi * L + r] = c128{qc * x.re - (qsr * y.re + qsi * y.im), qc * x.im - (qsr * y.im - qsi * y.re)};
-        J[qj * L + r] = c128{(qsr * x.re - qsi * x.im) + qc * y.re, (qsr * x.im + qsi * x.re) + qc * y.im};
+    if (round == L - 1) break;
+    if (has_g) {
+      const int pi = rr_fast(gP, round, L), pj = rr_fast(L - 1 - gP, round, L);
+      const int qi = rr_fast(gQ, round, L), qj = rr_fast(L - 1 - gQ, round, L);
+      const c128 t00 = G[qi * L + pi], t10 = G[qi * L + pj], t01 = G[qj * L + pi], t11 = G[qj * L + pj];
+      const JRot P = rot[cur][gP], Q = rot[cur][gQ];
+      if (P.act || Q.act) {
+        const c128 u00{Q.c * t00.re - (Q.sr * t01.re + Q.si * t01.im), Q.c * t00.im - (Q.sr * t01.im - Q.si * t01.re)};
+        const c128 u10{Q.c * t10.re - (Q.sr * t11.re + Q.si * t11.im), Q.c * t10.im - (Q.sr * t11.im - Q.si * t11.re)};
+        const c128 u01{(Q.sr * t00.re - Q.si * t00.im) + Q.c * t01.re, (Q.sr * t00.im + Q.si * t00.re) + Q.c * t01.im};
+        const c128 u11{(Q.sr * t10.re - Q.si * t10.im) + Q.c * t11.re, (Q.sr * t10.im + Q.si * t10.re) + Q.c * t11.im};
+        G[qi * L + pi] = c128{P.c * u00.re - (P.sr * u10.re - P.si * u10.im), P.c * u00.im - (P.sr * u10.im + P.si * u10.re)};
+        G[qi * L + pj] = c128{(P.sr * u00.re + P.si * u00.im) + P.c * u10.re, (P.sr * u00.im - P.si * u00.re) + P.c * u10.im};
+        G[qj * L + pi] = c128{P.c * u01.re - (P.sr * u11.re - P.si * u11.im), P.c * u01.im - (P.sr * u11.im + P.si * u11.re)};
+        G[qj * L + pj] = c128{(P.sr * u01.re + P.si * u01.im) + P.c * u11.re, (P.sr * u01.im - P.si * u01.re) + P.c * u11.im};
       }
     }
     __syncthreads();
