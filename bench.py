@@ -97,19 +97,46 @@ def heff2_roofline(torch, prec: int, reps: int = 50):
     return ms, flops
 
 
-def profiler_control(enabled: bool):
-    """Returns on(bool): resume/pause rocprofv3 collection (roctx profiler
-    control) when `enabled`, a no-op otherwise.  Collection starts paused."""
-    if not enabled:
-        return lambda on: None
-    path = "/opt/rocm/lib/librocprofiler-sdk-roctx.so"
-    lib = ctypes.CDLL(path if os.path.exists(path) else "librocprofiler-sdk-roctx.so")
-    lib.roctxProfilerPause.argtypes = [ctypes.c_int]
-    lib.roctxProfilerResume.argtypes = [ctypes.c_int]
-
-    def on(resume: bool) -> None:
-        (lib.roctxProfilerResume if resume else lib.roctxProfilerPause)(0)
-    return on
+def batch_bench(torch, a):
+    """Aggregate 2TDVP site-updates/s of the first `a.batch` instances of the
+    batched workload (mixed precision), advanced one step at a time by
+    `a.batch_threads` native workers, one stream per instance; the same engines
+    stepped one after another on one host thread give the sequential rate."""
+    import qca_native as qn
+    import workloads
+    insts = workloads.batch_instances(256, seed=4)[:a.batch]
+    log(f"batched workload: {len(insts)} instances, building engines")
+    engines = [qn.Engine(workloads.mpo_tensors(i), workloads.initial_tensors(i), algorithm=2, precision=1,
+                         chi_max=i.chi_max, dt=i.step_size, eps=i.svd_epsilon, krylov_tol=1e-12)
+               for i in insts]
+    qn.batch_step(engines, a.batch_warmup, a.batch_threads)
+    torch.cuda.synchronize()
+    bonds = [int(e.bond_dims().max()) for e in engines]
+    log(f"batched workload: warm after {a.batch_warmup} steps, max bond per instance "
+        f"median {int(np.median(bonds))} max {max(bonds)}")
+    rates = {}
+    for label, run in (("sequential", lambda: [e.step(a.batch_steps) for e in engines]),
+                       ("batched", lambda: qn.batch_step(engines, a.batch_steps, a.batch_threads))):
+        for e in engines:
+            e.reset_stats()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        su = sum(e.stats()["site_updates"] for e in engines)
+        rates[label] = su / dt
+        log(f"batched workload: {label} {su} site-updates in {dt:.2f} s")
+    for e in engines:
+        e.close()
+    return {"workload": f"first {len(insts)} instances of the batched 2TDVP draw (default_rng(4)): "
+                        "N~U{15..61}, chi_max~{16,32,64,128}, d~U{1,2}, 1<=a<=b<=2d, p=0.3, dt 0.1, "
+                        "eps 1e-10, mixed precision",
+            "instances": len(insts), "threads": a.batch_threads, "warmup_steps": a.batch_warmup,
+            "timed_steps": a.batch_steps, "max_bond_median": int(np.median(bonds)),
+            "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
+            "site_updates_per_s": rates["batched"], "sequential_site_updates_per_s": rates["sequential"],
+            "speedup_over_sequential": rates["batched"] / rates["sequential"]}
 
 
 def cpu_baseline(As, seconds: float):
@@ -147,9 +174,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--profile-timed", action="store_true",
-                    help="pause rocprofv3 collection outside the timed steps (run under "
-                         "rocprofv3 --selected-regions)")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="also time the first N instances of the batched 2TDVP workload "
+                         "(default 0: skip)")
+    ap.add_argument("--batch-threads", type=int, default=16)
+    ap.add_argument("--batch-warmup", type=int, default=10)
+    ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the H_eff micro-benchmark and print its roofline object "
                          "(the command the kernel profiles are taken with)")
@@ -158,6 +188,10 @@ def main() -> int:
         print("bench.py measures one GPU: every box in this pool has a single MI355X, and the "
               "2TDVP sweep of one instance does not shard; run with --gpus 1", file=sys.stderr)
         return 2
+    if a.batch:
+        # one hardware queue per worker stream (HIP's default is 4); must be set
+        # before the HIP runtime starts
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(16, max(4, a.batch_threads)))
     import torch
     import qca_native as qn
     if not torch.cuda.is_available():
@@ -186,7 +220,6 @@ def main() -> int:
     engines = {"mixed": qn.Engine(Ws, engines["fp64"].tensors(), algorithm=2, precision=1, chi_max=CHI,
                                   dt=DT, eps=EPS, krylov_tol=1e-12),
                "fp64": engines["fp64"]}
-    prof = profiler_control(a.profile_timed)
     maxdiff = 0.0
     times = {"mixed": 0.0, "fp64": 0.0}
     gpu_ms = {"mixed": 0.0, "fp64": 0.0}
@@ -196,12 +229,10 @@ def main() -> int:
     for k in range(a.steps):
         for name, e in engines.items():
             torch.cuda.synchronize()
-            prof(True)
             t0 = time.perf_counter()
             e.step(1)
             torch.cuda.synchronize()
             times[name] += time.perf_counter() - t0
-            prof(False)
         d = np.abs(engines["mixed"].rho()[:, 1, 1].real - engines["fp64"].rho()[:, 1, 1].real).max()
         maxdiff = max(maxdiff, d)
         log(f"timed step {k + 1}/{a.steps}: mixed {times['mixed']:.2f}s fp64 {times['fp64']:.2f}s |dn| {d:.2e}")
@@ -216,6 +247,7 @@ def main() -> int:
     ms64, _ = heff2_roofline(torch, 0)
     ach32 = fl / (ms32 * 1e-3) / 1e12
     ach64 = fl / (ms64 * 1e-3) / 1e12
+    batched = batch_bench(torch, a) if a.batch else None
     cpu = None
     if not a.no_cpu_baseline:
         log(f"cpu baseline ({a.cpu_seconds:.0f} s budget)")
@@ -252,6 +284,7 @@ def main() -> int:
                      "traffic": None, "ms_per_launch": ms32, "flops_per_launch": fl,
                      "fp64": {"achieved": ach64, "peak": 78.6, "frac": ach64 / 78.6, "ms_per_launch": ms64}},
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
     return 0

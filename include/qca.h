@@ -99,6 +99,17 @@ int qca_engine_reset_stats(qca_engine* e);
  * measured with HIP events around each qca_engine_step call. */
 int qca_engine_gpu_ms(qca_engine* e, double* ms);
 
+/* Advances each of the n engines by nsteps time steps using nthreads host
+ * worker threads (1..256).  Each engine runs on its own stream, so up to
+ * nthreads instances are in flight on the GPU at once; the workers take engines
+ * from a round-robin queue one time step at a time, so instances of unequal
+ * cost stay balanced.  Equivalent to calling qca_engine_step(engines[i], nsteps)
+ * for every i.  On failure returns the first error, stops handing out work and
+ * sets *failed (if not NULL) to the failing engine's index, else to -1.
+ * No counterpart in the reference, which runs one simulation per process
+ * (quantum_game.py:75-119 loops over initial states sequentially). */
+int qca_batch_step(qca_engine* const* engines, int n, int nsteps, int nthreads, int* failed);
+
 /* ------------------------------------------------------------------ primitives */
 
 /* y = H_eff x (matrix-free).  Replaces the dense assembly + product of

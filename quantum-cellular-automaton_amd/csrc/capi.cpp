@@ -4,8 +4,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <map>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "engine.hpp"
@@ -152,6 +155,21 @@ struct qca_engine {
   double gpu_ms = 0;
 };
 
+namespace {
+
+// One time step of `e` on its stream, with its GPU time added to e->gpu_ms.
+void engine_one_step(qca_engine* e) {
+  QCA_HIP(hipEventRecord(e->ev0, e->eng->stream()));
+  e->eng->step();
+  QCA_HIP(hipEventRecord(e->ev1, e->eng->stream()));
+  QCA_HIP(hipEventSynchronize(e->ev1));
+  float ms = 0;
+  QCA_HIP(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+  e->gpu_ms += ms;
+}
+
+}  // namespace
+
 extern "C" {
 
 int qca_version(void) { return 10000; }
@@ -214,16 +232,64 @@ int qca_engine_step(qca_engine* e, int nsteps) {
   return guarded([&] {
     QCA_CHECK(e && e->eng, "null engine");
     QCA_CHECK(nsteps >= 0, "negative step count");
-    for (int i = 0; i < nsteps; ++i) {
-      QCA_HIP(hipEventRecord(e->ev0, e->eng->stream()));
-      e->eng->step();
-      QCA_HIP(hipEventRecord(e->ev1, e->eng->stream()));
-      QCA_HIP(hipEventSynchronize(e->ev1));
-      float ms = 0;
-      QCA_HIP(hipEventElapsedTime(&ms, e->ev0, e->ev1));
-      e->gpu_ms += ms;
-    }
+    QCA_HIP(hipSetDevice(e->eng->device()));
+    for (int i = 0; i < nsteps; ++i) engine_one_step(e);
   });
+}
+
+int qca_batch_step(qca_engine* const* engines, int n, int nsteps, int nthreads, int* failed) {
+  std::string first_error;
+  int first_code = QCA_OK;
+  const int rc = guarded([&] {
+    QCA_CHECK(engines != nullptr || n == 0, "null engine list");
+    QCA_CHECK(n >= 0 && nsteps >= 0, "negative count");
+    QCA_CHECK(nthreads >= 1 && nthreads <= 256, "nthreads out of range (1..256)");
+    for (int i = 0; i < n; ++i) QCA_CHECK(engines[i] && engines[i]->eng, "null engine in batch");
+    if (failed) *failed = -1;
+    if (n == 0 || nsteps == 0) return;
+    // round-robin queue of (engine, steps left): a worker takes the front
+    // engine, advances it one time step on its own stream and requeues it
+    std::mutex mu;
+    std::deque<std::pair<int, int>> queue;
+    for (int i = 0; i < n; ++i) queue.emplace_back(i, nsteps);
+    auto worker = [&] {
+      while (true) {
+        std::pair<int, int> job;
+        {
+          std::lock_guard<std::mutex> lock(mu);
+          if (queue.empty() || first_code != QCA_OK) return;
+          job = queue.front();
+          queue.pop_front();
+        }
+        qca_engine* e = engines[job.first];
+        const int code = guarded([&] {
+          QCA_HIP(hipSetDevice(e->eng->device()));
+          engine_one_step(e);
+        });
+        std::lock_guard<std::mutex> lock(mu);
+        if (code != QCA_OK) {
+          if (first_code == QCA_OK) {
+            first_code = code;
+            first_error = "engine " + std::to_string(job.first) + ": " + g_last_error;
+            if (failed) *failed = job.first;
+          }
+          return;
+        }
+        if (job.second > 1) queue.emplace_back(job.first, job.second - 1);
+      }
+    };
+    const int nt = std::min(nthreads, n);
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto& th : pool) th.join();
+  });
+  if (rc != QCA_OK) return rc;
+  if (first_code != QCA_OK) {
+    g_last_error = first_error;
+    return first_code;
+  }
+  return QCA_OK;
 }
 
 int qca_engine_measure(qca_engine* e, double* rho_out) {

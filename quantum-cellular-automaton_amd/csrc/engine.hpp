@@ -47,6 +47,7 @@ class Engine {
   void measure(c128* rho_host);                       // 4 complex per site
   int chi(int bond) const { return chi_[bond]; }
   int ncells() const { return N_; }
+  int device() const { return device_; }
   void get_site(int i, c128* host) const;             // copies (2, chi_i, chi_{i+1})
   void set_stream(hipStream_t s) { s_ = s; }
   hipStream_t stream() const { return s_; }

@@ -150,31 +150,13 @@ __device__ __forceinline__ int rr_fast(int t, int r, int n) {
   x -= (x >= n - 1) ? (n - 1) : 0;
   return x + 1;
 }
-__global__ __launch_bounds__(JAC_NT) void jac_inner(JacobiGeom g, int sweep,
-                                                    const c128* __restrict__ Gpart,
-                                                    c128* __restrict__ Jout,
-                                                    int* __restrict__ rotated, double tol,
-                                                    JacobiCtrl* __restrict__ ctrl) {
-  if (ctrl->done) return;
-  constexpr int HM = JAC_LMAX / 2;
-  __shared__ c128 G[JAC_LMAX * JAC_LMAX];
-  __shared__ c128 J[JAC_LMAX * JAC_LMAX];
-  __shared__ JRot rot[2][HM];
-  __shared__ int total;
-  const int p = blockIdx.x, L = 2 * g.bs, H = g.bs, tid = threadIdx.x;
-  const double negl = 1e-30 * ctrl->fro;
-  const double tol2 = tol * tol;
-  for (int e = tid; e < L * L; e += JAC_NT) {
-    double re = 0, im = 0;
-    for (int kc = 0; kc < g.kspl; ++kc) {
-      const c128 v = Gpart[((long)p * g.kspl + kc) * L * L + e];
-      re += v.re;
-      im += v.im;
-    }
-    G[e] = c128{re, im};
-    J[e] = c128{(e % L) == (e / L) ? 1.0 : 0.0, 0.0};
-  }
-  if (tid == 0) total = 0;
+// One cyclic sweep of the rotations on G (L x L, column-major, in LDS),
+// accumulating J <- J R; rot is a 2 x (L/2) LDS scratch.  Needs blockDim.x ==
+// JAC_NT and G, J visible to the whole workgroup on entry (it ends with a
+// barrier).  Returns the rotations this thread computed.
+__device__ int inner_sweep(c128* G, c128* J, JRot (*rot)[JAC_LMAX / 2], int L, int H, double tol2,
+                           double negl) {
+  const int tid = threadIdx.x;
   int mine = 0;
   // fixed roles: lanes < H of wave 0 compute the rotations; threads 64.. own
   // one J item (row r, pair Q) each; threads < H*H own one 2x2 block of G
@@ -183,7 +165,6 @@ __global__ __launch_bounds__(JAC_NT) void jac_inner(JacobiGeom g, int sweep,
   const int jr = has_j ? jit % L : 0, jq = has_j ? jit / L : 0;
   const bool has_g = tid < H * H;
   const int gP = tid % H, gQ = tid / H;
-  __syncthreads();
   for (int round = 0; round < L; ++round) {
     const int cur = round & 1, prev = cur ^ 1;
     if (tid < 64) {
@@ -235,6 +216,35 @@ __global__ __launch_bounds__(JAC_NT) void jac_inner(JacobiGeom g, int sweep,
     }
     __syncthreads();
   }
+  return mine;
+}
+
+__global__ __launch_bounds__(JAC_NT) void jac_inner(JacobiGeom g, int sweep,
+                                                    const c128* __restrict__ Gpart,
+                                                    c128* __restrict__ Jout,
+                                                    int* __restrict__ rotated, double tol,
+                                                    JacobiCtrl* __restrict__ ctrl) {
+  if (ctrl->done) return;
+  constexpr int HM = JAC_LMAX / 2;
+  __shared__ c128 G[JAC_LMAX * JAC_LMAX];
+  __shared__ c128 J[JAC_LMAX * JAC_LMAX];
+  __shared__ JRot rot[2][HM];
+  __shared__ int total;
+  const int p = blockIdx.x, L = 2 * g.bs, H = g.bs, tid = threadIdx.x;
+  const double negl = 1e-30 * ctrl->fro;
+  for (int e = tid; e < L * L; e += JAC_NT) {
+    double re = 0, im = 0;
+    for (int kc = 0; kc < g.kspl; ++kc) {
+      const c128 v = Gpart[((long)p * g.kspl + kc) * L * L + e];
+      re += v.re;
+      im += v.im;
+    }
+    G[e] = c128{re, im};
+    J[e] = c128{(e % L) == (e / L) ? 1.0 : 0.0, 0.0};
+  }
+  if (tid == 0) total = 0;
+  __syncthreads();
+  const int mine = inner_sweep(G, J, rot, L, H, tol * tol, negl);
   if (mine) atomicAdd(&total, mine);
   __syncthreads();
   c128* jo = Jout + (long)p * L * L;
@@ -280,13 +290,11 @@ __global__ void jac_check(JacobiCtrl* __restrict__ ctrl, int sweep) {
   if (ctrl->rot[sweep] == 0) ctrl->done = 1;
 }
 
-// Column norms, descending sort and truncation rank (one workgroup).
-__global__ __launch_bounds__(JAC_NT) void jac_finalize(const c128* __restrict__ X, int rows, int cols,
-                                                       double* __restrict__ sigma,
-                                                       int* __restrict__ perm,
-                                                       SvdResult* __restrict__ res, int chi_max,
-                                                       double eps, int rank_cap,
-                                                       const JacobiCtrl* __restrict__ ctrl) {
+// Column norms, descending sort and truncation rank (one workgroup of JAC_NT).
+__device__ void finalize_body(const c128* __restrict__ X, int rows, int cols,
+                              double* __restrict__ sigma, int* __restrict__ perm,
+                              SvdResult* __restrict__ res, int chi_max, double eps, int rank_cap,
+                              int sweeps, int converged) {
   __shared__ double skey[JAC_MAXCOLS];
   __shared__ int sidx[JAC_MAXCOLS];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = JAC_NT / 64;
@@ -344,11 +352,101 @@ __global__ __launch_bounds__(JAC_NT) void jac_finalize(const c128* __restrict__ 
     for (int e = 0; e < k; ++e) kept += skey[e] * skey[e];
     for (int e = k; e < ns; ++e) tail += skey[e] * skey[e];
     res->k = k;
-    res->nsweeps = ctrl->sweeps;
-    res->converged = ctrl->done;
+    res->nsweeps = sweeps;
+    res->converged = converged;
     res->norm_kept = sqrt(kept);
     res->tail = sqrt(tail);
   }
+}
+
+__global__ __launch_bounds__(JAC_NT) void jac_finalize(const c128* __restrict__ X, int rows, int cols,
+                                                       double* __restrict__ sigma,
+                                                       int* __restrict__ perm,
+                                                       SvdResult* __restrict__ res, int chi_max,
+                                                       double eps, int rank_cap,
+                                                       const JacobiCtrl* __restrict__ ctrl) {
+  finalize_body(X, rows, cols, sigma, perm, res, chi_max, eps, rank_cap, ctrl->sweeps, ctrl->done);
+}
+
+// The whole SVD of a matrix with at most JAC_LMAX columns in one workgroup:
+// sweeps of (Gram from the columns -> one rotation sweep on it -> X <- X J)
+// until a sweep makes no rotation, then finalize_body.  Same rotations, test
+// and accuracy as the kernel triple, without launches between rounds.
+__global__ __launch_bounds__(JAC_NT) void jac_small(c128* __restrict__ X, int rows, int cols,
+                                                    double* __restrict__ sigma,
+                                                    int* __restrict__ perm,
+                                                    SvdResult* __restrict__ res, int chi_max,
+                                                    double eps, int rank_cap, double tol) {
+  constexpr int HM = JAC_LMAX / 2;
+  __shared__ c128 tile[JAC_RC * (JAC_LMAX + 1)];
+  __shared__ c128 G[JAC_LMAX * JAC_LMAX];
+  __shared__ c128 J[JAC_LMAX * JAC_LMAX];
+  __shared__ JRot rot[2][HM];
+  __shared__ double red[JAC_NT / 64];
+  __shared__ int total;
+  const int tid = threadIdx.x;
+  const int H = (cols + 1) / 2, L = 2 * H;
+  double part = 0;
+  for (long e = tid; e < (long)rows * cols; e += JAC_NT) part += X[e].re * X[e].re + X[e].im * X[e].im;
+  part = wsum(part);
+  if ((tid & 63) == 0) red[tid >> 6] = part;
+  __syncthreads();
+  double fro = 0;
+  for (int w = 0; w < JAC_NT / 64; ++w) fro += red[w];
+  const double negl = 1e-30 * fro;
+  const JacobiGeom g{rows, cols, H, 2, 1};  // one block pair: columns 0..H-1 and H..L-1
+  int sweep = 0, converged = 0;
+  for (; sweep < JAC_MAX_SWEEPS; ++sweep) {
+    // Gram from the current columns, 64-row chunks through the tile
+    double gre = 0, gim = 0;
+    const int gi = tid % L, gj = tid / L;
+    for (int r0 = 0; r0 < rows; r0 += JAC_RC) {
+      const int nr = min(JAC_RC, rows - r0);
+      __syncthreads();
+      stage_tile(X, g, 0, 0, r0, nr, tile);
+      __syncthreads();
+      if (tid < L * L)
+        for (int r = 0; r < nr; ++r) {
+          const c128 u = tile[r * (JAC_LMAX + 1) + gi], v = tile[r * (JAC_LMAX + 1) + gj];
+          gre += u.re * v.re + u.im * v.im;
+          gim += u.re * v.im - u.im * v.re;
+        }
+    }
+    if (tid < L * L) {
+      G[tid] = c128{gre, gim};
+      J[tid] = c128{gi == gj ? 1.0 : 0.0, 0.0};
+    }
+    if (tid == 0) total = 0;
+    __syncthreads();
+    const int mine = inner_sweep(G, J, rot, L, H, tol * tol, negl);
+    if (mine) atomicAdd(&total, mine);
+    __syncthreads();
+    if (total == 0) {
+      converged = 1;
+      break;
+    }
+    // X <- X J, chunk by chunk (each chunk is read into the tile before it is written)
+    for (int r0 = 0; r0 < rows; r0 += JAC_RC) {
+      const int nr = min(JAC_RC, rows - r0);
+      stage_tile(X, g, 0, 0, r0, nr, tile);
+      __syncthreads();
+      for (int e = tid; e < L * nr; e += JAC_NT) {
+        const int c = e / nr, r = e - c * nr;
+        if (c >= cols) continue;
+        double re = 0, im = 0;
+        for (int k = 0; k < L; ++k) {
+          const c128 x = tile[r * (JAC_LMAX + 1) + k], jj = J[c * L + k];
+          re += x.re * jj.re - x.im * jj.im;
+          im += x.re * jj.im + x.im * jj.re;
+        }
+        X[(long)c * rows + r0 + r] = c128{re, im};
+      }
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  finalize_body(X, rows, cols, sigma, perm, res, chi_max, eps, rank_cap,
+                converged ? sweep + 1 : sweep, converged);
 }
 
 // ------------------------------------------------------------------ QR
@@ -490,6 +588,13 @@ void launch_jacobi_svd(c128* X, int rows, int cols, double* sigma, int* perm, Sv
   QCA_CHECK(rows >= 1 && rows <= 512, "SVD rows out of range (1..512)");
   QCA_CHECK(cols >= 1 && cols <= JAC_MAXCOLS, "SVD cols out of range (1..512)");
   QCA_CHECK(w.ws != nullptr && w.host != nullptr, "Jacobi workspace not set");
+  const double tol = 1.1102230246251565e-16 * sqrt((double)rows);
+  if (cols <= JAC_LMAX) {
+    hipLaunchKernelGGL(jac_small, dim3(1), dim3(JAC_NT), 0, s, X, rows, cols, sigma, perm, res, chi_max,
+                       eps, rank_cap, tol);
+    QCA_HIP(hipGetLastError());
+    return;
+  }
   const JacobiLayout l = jacobi_layout(rows, cols);
   QCA_CHECK(l.bytes <= w.ws_bytes, "Jacobi workspace too small");
   static_assert(sizeof(JacobiCtrl) <= 256, "control block exceeds its slot");
@@ -502,7 +607,6 @@ void launch_jacobi_svd(c128* X, int rows, int cols, double* sigma, int* perm, Sv
             "Jacobi workspace too small");
   const JacobiGeom g{rows, cols, l.bs, l.nb, l.kspl};
   const int np = l.nb / 2;
-  const double tol = 1.1102230246251565e-16 * sqrt((double)rows);
   hipLaunchKernelGGL(jac_init, dim3(1), dim3(JAC_NT), 0, s, X, (long)rows * cols, ctrl);
   // launch the sweeps predicted from the previous call, then read the device
   // flag once per chunk (rounds after convergence are no-ops)

@@ -50,6 +50,7 @@ SIGNATURES = {
     "qca_engine_get_site": (c_int, [c_void_p, c_int, c_void_p]),
     "qca_engine_stats": (c_int, [c_void_p, P(Stats)]),
     "qca_engine_reset_stats": (c_int, [c_void_p]),
+    "qca_batch_step": (c_int, [P(c_void_p), c_int, c_int, c_int, P(c_int)]),
     "qca_engine_gpu_ms": (c_int, [c_void_p, P(c_double)]),
     "qca_heff_apply": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_int, c_int, c_int, c_int, c_void_p]),
@@ -187,3 +188,15 @@ class Engine:
         v = c_double(0)
         check(lib().qca_engine_gpu_ms(self._h, ctypes.byref(v)))
         return v.value
+
+
+def batch_step(engines, nsteps: int = 1, nthreads: int = 8) -> None:
+    """Advance every engine by ``nsteps`` time steps with ``nthreads`` native
+    worker threads, one stream per engine (qca_batch_step)."""
+    engines = list(engines)
+    if not engines:
+        return
+    hs = (c_void_p * len(engines))(*[e._h for e in engines])
+    failed = c_int(-1)
+    rc = lib().qca_batch_step(hs, len(engines), nsteps, nthreads, ctypes.byref(failed))
+    check(rc)

@@ -41,3 +41,13 @@ def test_argument_errors_are_reported_without_a_device():
     assert lib.qca_heff_apply(3, 0, None, None, None, None, None, 1, 1, 1, 1, None) == -2
     assert b"precision" in lib.qca_last_error()
     assert lib.qca_exact_expm(None, 10, 1, 1, 2, 0.1, 0.0, None, None) == -2
+    failed = ctypes.c_int(7)
+    assert lib.qca_batch_step(None, 2, 1, 4, ctypes.byref(failed)) == -2
+    assert b"null engine list" in lib.qca_last_error()
+    hs = (ctypes.c_void_p * 1)(None)
+    assert lib.qca_batch_step(hs, 1, 1, 0, None) == -2
+    assert b"nthreads" in lib.qca_last_error()
+    assert lib.qca_batch_step(hs, 1, 1, 2, None) == -2
+    assert b"null engine in batch" in lib.qca_last_error()
+    assert lib.qca_batch_step(None, 0, 5, 1, ctypes.byref(failed)) == 0
+    assert failed.value == -1

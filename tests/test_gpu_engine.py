@@ -218,3 +218,32 @@ def test_driver_cadence_and_files(gpu, tmp_path):
                       gauge="consistent", expm="lanczos")
     ref = OT.run_trajectory(o, 12, 2)
     assert np.abs(out["population"] - ref["population"]).max() < 1e-10
+
+
+def test_batch_step_equals_sequential(gpu):
+    """qca_batch_step over independent engines (one stream each, 4 host
+    workers) gives bit-identical states to stepping each engine alone: the
+    kernels are deterministic and engines share no device state."""
+    import workloads
+    insts = workloads.batch_instances(6, seed=4)
+    def engines(prec):
+        return [make_engine(workloads.initial_tensors(i), workloads.mpo_tensors(i), 2, min(i.chi_max, 32),
+                            i.step_size, i.svd_epsilon, prec=prec) for i in insts]
+    for prec in (0, 1):
+        batched, alone = engines(prec), engines(prec)
+        qn.batch_step(batched, 4, nthreads=4)
+        for e in alone:
+            e.step(4)
+        for b, a in zip(batched, alone):
+            assert np.array_equal(b.bond_dims(), a.bond_dims())
+            assert np.array_equal(b.rho(), a.rho())
+            assert b.stats()["site_updates"] == a.stats()["site_updates"]
+    # one batched instance against the oracle
+    i = insts[0]
+    e = make_engine(workloads.initial_tensors(i), workloads.mpo_tensors(i), 2, 16, 0.1, 1e-10)
+    qn.batch_step([e], 3, nthreads=2)
+    o = OT.TDVPOracle(OM.product_state(i.plist), AU.mpo_tensors(i.ncells, i.distance, i.interval), "2tdvp",
+                      0.1, 16, 1e-10, gauge="consistent", expm="lanczos")
+    for _ in range(3):
+        o.do_time_step()
+    assert np.abs(e.rho()[:, 1, 1].real - OM.measure_consistent(o.A)[0]).max() < 1e-10

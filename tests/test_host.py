@@ -79,3 +79,39 @@ def test_classical_overlay():
     h = Algorithm.classical_evolution(first, rules, 3)
     assert list(h[1]) == [0, 0, 1, 1, 1, 0, 0]
     assert list(h[2]) == [0, 1, 0, 1, 0, 1, 0]
+
+
+def test_batch_workload_draws():
+    """Batched workload: N ~ U{15..61}, chi ~ {16,32,64,128}, d ~ U{1,2},
+    1 <= a <= b <= 2d, cells alive with probability 0.3, reproducible."""
+    import workloads
+    a = workloads.batch_instances(256, seed=4)
+    b = workloads.batch_instances(256, seed=4)
+    assert [(i.ncells, i.chi_max, i.distance, i.interval, i.plist) for i in a] == \
+           [(i.ncells, i.chi_max, i.distance, i.interval, i.plist) for i in b]
+    assert {i.chi_max for i in a} == {16, 32, 64, 128}
+    assert {i.distance for i in a} == {1, 2}
+    for i in a:
+        assert 15 <= i.ncells <= 61 and len(i.plist) == i.ncells
+        assert 1 <= i.interval.start <= i.interval.stop - 1 <= 2 * i.distance
+        assert set(i.plist) <= {0.0, 1.0}
+        assert i.site_updates_per_step == 2 * (i.ncells - 1)
+    frac = np.mean([p for i in a for p in i.plist])
+    assert 0.25 < frac < 0.35
+    # prefixes are stable: the first k draws do not depend on the count
+    assert [i.ncells for i in workloads.batch_instances(10, seed=4)] == [i.ncells for i in a[:10]]
+    W = workloads.mpo_tensors(a[0])
+    assert len(W) == a[0].ncells and W[0].shape[2] == 1
+    A = workloads.initial_tensors(a[0])
+    assert len(A) == a[0].ncells and A[0].shape == (2, 1, 1)
+
+
+def test_serving_request_draws():
+    import workloads
+    rng = np.random.default_rng(5)
+    reqs = [workloads.serving_request(rng) for _ in range(400)]
+    big = [r for r, s in reqs if r.ncells == 101]
+    assert 20 <= len(big) <= 60
+    for r, s in reqs:
+        assert 50 <= s <= 500
+    assert all(r.chi_max == 256 and r.interval == range(2, 4) for r in big)
