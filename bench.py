@@ -42,6 +42,13 @@ for p in (ROOT, PKG):
 
 N, DIST, IV, DT, CHI, EPS = 41, 2, range(2, 5), 0.1, 128, 1e-10
 
+# HBM bytes moved by one 2-site H_eff application at chi=128, D=13, measured with
+# rocprofv3 FETCH_SIZE and WRITE_SIZE passes over `bench.py --roofline-only`
+# (the counters scaled by a 256 MiB device copy in the same runs).  The
+# algorithmic minimum is 4.46 MB (c64); the rest is the L2-spilling
+# intermediates and split-K partials.
+HEFF2_TRAFFIC_BYTES = {"mixed": 72.81e6, "fp64": 145.60e6}
+
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -200,6 +207,12 @@ def main() -> int:
     if a.roofline_only:
         ms32, fl = heff2_roofline(torch, 1)
         ms64, _ = heff2_roofline(torch, 0)
+        # a plain device copy of known size (256 MiB read + 256 MiB written):
+        # the scale for HBM byte counters read in the same run
+        src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
         print(json.dumps({"roofline": {"mixed_ms_per_launch": ms32, "fp64_ms_per_launch": ms64,
                                        "flops_per_launch": fl,
                                        "mixed_tflops": fl / (ms32 * 1e-3) / 1e12,
@@ -281,8 +294,12 @@ def main() -> int:
         "contraction_tflops_per_s": {k: flops[k]["flops"] / times[k] / 1e12 for k in flops},
         "roofline": {"bound": "mfma", "kernel": "2-site H_eff application (mixed, complex64 on FP32 MFMA)",
                      "achieved": ach32, "peak": 157.3, "unit": "TFLOP/s", "frac": ach32 / 157.3,
-                     "traffic": None, "ms_per_launch": ms32, "flops_per_launch": fl,
-                     "fp64": {"achieved": ach64, "peak": 78.6, "frac": ach64 / 78.6, "ms_per_launch": ms64}},
+                     "traffic": HEFF2_TRAFFIC_BYTES["mixed"], "traffic_unit": "B per launch",
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --roofline-only, "
+                                       "scaled by a 256 MiB copy (profiles/heff2_chi128_d13_hbm_bytes.md)",
+                     "ms_per_launch": ms32, "flops_per_launch": fl,
+                     "fp64": {"achieved": ach64, "peak": 78.6, "frac": ach64 / 78.6, "ms_per_launch": ms64,
+                              "traffic": HEFF2_TRAFFIC_BYTES["fp64"]}},
         "cpu_baseline": cpu,
         "batched": batched,
     }

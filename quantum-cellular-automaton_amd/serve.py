@@ -1,0 +1,191 @@
+"""Serving many independent TDVP simulations on one GPU: Poisson arrivals,
+admission by HBM footprint, continuous batching, per-request latency.
+
+A request is one simulation (an `workloads.Instance` plus a step count); the
+server returns <n_i> every `report_every` steps, as the reference driver
+measures every plotting interval (quantum_game.py:82-119), and the request's
+latency is the time from its arrival to its last returned <n_i>.
+
+Scheduling: admitted requests sit in a round-robin queue; `workers` host
+threads repeatedly take the front request, advance it by one reporting chunk
+on its own engine (its own HIP stream), measure, and requeue it until done.
+New arrivals join the queue while others run (continuous batching), so a short
+request is never stuck behind a long one.  A request is admitted when its
+estimated device footprint fits in the remaining budget; otherwise it waits in
+arrival order.  The engine factory is injectable, so the scheduler is tested
+on the host with a stand-in engine.
+"""
+from __future__ import annotations
+
+import collections
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+import workloads
+
+
+def footprint_bytes(inst: workloads.Instance, precision: int = 1) -> int:
+    """Upper bound of an engine's device memory: MPS (c128), environments
+    (c64 mixed / c128 plain), Krylov basis (64 vectors of the largest 2-site
+    block) and per-engine workspaces."""
+    n, chi = inst.ncells, inst.chi_max
+    dims = [min(2 ** min(i, 40), 2 ** min(n - i, 40), chi) for i in range(n + 1)]
+    D = 2 + 2 * inst.distance + 4 * inst.distance * inst.distance  # generous MPO bond bound
+    env_elem = 8 if precision == 1 else 16
+    mps = sum(2 * dims[i] * dims[i + 1] for i in range(n)) * 16
+    env = 2 * sum(dims[i] ** 2 * D for i in range(n + 1)) * env_elem
+    nmax = 4 * chi * chi
+    krylov = 66 * nmax * 16
+    work = 8 * nmax * D * 16 + (2 << 20)
+    return int(mps + env + krylov + work)
+
+
+@dataclass
+class Request:
+    rid: int
+    inst: workloads.Instance
+    steps: int
+    arrival: float
+    done_steps: int = 0
+    engine: object = None
+    reports: list = field(default_factory=list)
+    finished: float | None = None
+    footprint: int = 0
+
+
+class Server:
+    def __init__(self, make_engine, workers: int = 8, budget_bytes: int = 200 << 30,
+                 report_every: int = 10, precision: int = 1):
+        self.make_engine = make_engine
+        self.workers = workers
+        self.budget = budget_bytes
+        self.report_every = report_every
+        self.precision = precision
+        self._cv = threading.Condition()
+        self._pending = collections.deque()   # arrived, not admitted (arrival order)
+        self._ready = collections.deque()     # admitted, waiting for a worker
+        self._in_use = 0
+        self.peak_in_use = 0
+        self._closed = False
+        self._error = None
+        self.finished: list = []
+        self.site_updates = 0
+
+    # ------------------------------------------------------------ intake
+    def submit(self, req: Request) -> None:
+        req.footprint = footprint_bytes(req.inst, self.precision)
+        if req.footprint > self.budget:
+            raise ValueError(f"request {req.rid} needs {req.footprint} B, above the budget")
+        with self._cv:
+            self._pending.append(req)
+            self._admit()
+            self._cv.notify_all()
+
+    def close(self) -> None:
+        """No more arrivals; workers exit when the queues drain."""
+        with self._cv:
+            self._closed = True
+            self._cv.notify_all()
+
+    def _admit(self) -> None:
+        while self._pending and self._in_use + self._pending[0].footprint <= self.budget:
+            req = self._pending.popleft()
+            self._in_use += req.footprint
+            self.peak_in_use = max(self.peak_in_use, self._in_use)
+            self._ready.append(req)
+
+    # ------------------------------------------------------------ workers
+    def _worker(self) -> None:
+        while True:
+            with self._cv:
+                while not self._ready and not (self._closed and not self._pending) and self._error is None:
+                    self._cv.wait()
+                if self._error is not None or not self._ready:
+                    return
+                req = self._ready.popleft()
+            try:
+                if req.engine is None:
+                    req.engine = self.make_engine(req.inst)
+                n = min(self.report_every, req.steps - req.done_steps)
+                req.engine.step(n)
+                req.done_steps += n
+                req.reports.append(np.asarray(req.engine.rho()[:, 1, 1].real))
+                finished = req.done_steps >= req.steps
+                if finished:
+                    req.finished = time.perf_counter()
+                    req.engine.close()
+                    req.engine = None
+            except Exception as exc:  # stop the server, keep the first error
+                with self._cv:
+                    if self._error is None:
+                        self._error = exc
+                    self._cv.notify_all()
+                return
+            with self._cv:
+                self.site_updates += n * req.inst.site_updates_per_step
+                if finished:
+                    self.finished.append(req)
+                    self._in_use -= req.footprint
+                    self._admit()
+                else:
+                    self._ready.append(req)
+                self._cv.notify_all()
+
+    def run(self, arrivals: list) -> dict:
+        """Feeds (arrival_offset_s, Request) pairs at their times, serves them
+        until all finish, and returns latency and throughput statistics."""
+        threads = [threading.Thread(target=self._worker, daemon=True) for _ in range(self.workers)]
+        t0 = time.perf_counter()
+        for t in threads:
+            t.start()
+        for offset, req in arrivals:
+            delay = t0 + offset - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            req.arrival = time.perf_counter()
+            self.submit(req)
+        self.close()
+        for t in threads:
+            t.join()
+        if self._error is not None:
+            raise self._error
+        wall = time.perf_counter() - t0
+        lat = np.array([r.finished - r.arrival for r in self.finished])
+        return {"requests": len(self.finished), "wall_s": wall,
+                "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
+                "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
+                "site_updates": self.site_updates,
+                "site_updates_per_s": self.site_updates / wall if wall > 0 else 0.0}
+
+
+def poisson_arrivals(rate: float, window_s: float, seed: int = 5, big_fraction: float = 0.1,
+                     max_steps: int | None = None) -> list:
+    """(offset_s, Request) pairs: exponential inter-arrival times of mean
+    1/rate within [0, window_s); each request is workloads.serving_request
+    (steps optionally capped at max_steps)."""
+    rng = np.random.default_rng(seed)
+    out, t, rid = [], 0.0, 0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= window_s:
+            return out
+        inst, steps = workloads.serving_request(rng, big_fraction)
+        if max_steps is not None:
+            steps = min(steps, max_steps)
+        out.append((t, Request(rid, inst, steps, 0.0)))
+        rid += 1
+
+
+def gpu_engine_factory(precision: int = 1, chi_cap: int | None = None):
+    """Engine factory for the GPU path: one qca_engine per request."""
+    import qca_native as qn
+
+    def make(inst: workloads.Instance):
+        chi = inst.chi_max if chi_cap is None else min(inst.chi_max, chi_cap)
+        return qn.Engine(workloads.mpo_tensors(inst), workloads.initial_tensors(inst), algorithm=2,
+                         precision=precision, chi_max=chi, dt=inst.step_size, eps=inst.svd_epsilon,
+                         krylov_tol=1e-12 if precision == 1 else 1e-14)
+    return make

@@ -115,3 +115,73 @@ def test_serving_request_draws():
     for r, s in reqs:
         assert 50 <= s <= 500
     assert all(r.chi_max == 256 and r.interval == range(2, 4) for r in big)
+
+
+class _FakeEngine:
+    """Host stand-in for a qca engine: fixed cost per step, <n_i> = steps done."""
+    live = 0
+    peak = 0
+    lock = None
+
+    def __init__(self, inst, cost):
+        import threading
+        self.n, self.cost, self.t = inst.ncells, cost, 0
+        with _FakeEngine.lock:
+            _FakeEngine.live += 1
+            _FakeEngine.peak = max(_FakeEngine.peak, _FakeEngine.live)
+
+    def step(self, k):
+        import time
+        time.sleep(self.cost * k)
+        self.t += k
+
+    def rho(self):
+        r = np.zeros((self.n, 2, 2), complex)
+        r[:, 1, 1] = self.t
+        return r
+
+    def close(self):
+        with _FakeEngine.lock:
+            _FakeEngine.live -= 1
+
+
+def test_server_schedules_every_request():
+    """Continuous batching on the host: every request gets <n> every 10 steps
+    and its last report after exactly `steps` steps; admission keeps the
+    footprint of live engines within the budget; latency statistics exist."""
+    import threading
+    import serve
+    _FakeEngine.lock = threading.Lock()
+    _FakeEngine.live = _FakeEngine.peak = 0
+    arrivals = serve.poisson_arrivals(rate=200.0, window_s=0.2, seed=5, max_steps=60)
+    assert len(arrivals) > 10
+    sizes = sorted(serve.footprint_bytes(r.inst) for _, r in arrivals)
+    budget = sizes[-1] + sizes[-2]          # at most a few requests at once
+    srv = serve.Server(lambda inst: _FakeEngine(inst, 2e-5), workers=4, budget_bytes=budget)
+    out = srv.run(arrivals)
+    assert out["requests"] == len(arrivals)
+    for _, r in arrivals:
+        assert len(r.reports) == -(-r.steps // 10)
+        assert r.reports[-1][0] == r.steps and r.engine is None
+        assert r.finished >= r.arrival
+    assert out["p99_latency_s"] >= out["p50_latency_s"] > 0
+    assert out["site_updates"] == sum(r.steps * r.inst.site_updates_per_step for _, r in arrivals)
+    assert _FakeEngine.live == 0
+    assert srv._in_use == 0 and 0 < srv.peak_in_use <= budget
+
+
+def test_server_rejects_oversized_and_reports_errors():
+    import serve
+    import workloads
+    inst = workloads.Instance(101, 256, 2, range(2, 4), [0.0] * 101)
+    srv = serve.Server(lambda i: None, workers=1, budget_bytes=1 << 20)
+    with pytest.raises(ValueError):
+        srv.submit(serve.Request(0, inst, 10, 0.0))
+
+    class Boom:
+        def __init__(self, inst):
+            raise RuntimeError("device lost")
+    small = workloads.batch_instances(2, seed=4)
+    srv = serve.Server(Boom, workers=2)
+    with pytest.raises(RuntimeError, match="device lost"):
+        srv.run([(0.0, serve.Request(i, s, 20, 0.0)) for i, s in enumerate(small)])
