@@ -61,7 +61,9 @@ typedef struct qca_engine qca_engine;
 typedef struct qca_engine_params {
   int device;        /* HIP device ordinal */
   int ncells;        /* number of sites N >= 2 */
-  int algorithm;     /* 1 = 1TDVP, 2 = 2TDVP (reference args.algorithm) */
+  int algorithm;     /* 1 = 1TDVP, 2 = 2TDVP, 3 = adaptive 1TDVP (reference
+                        args.algorithm 1tdvp / 2tdvp / a1tdvp; a1tdvp targets use the
+                        exact_exponential propagator, tdvp.py:212-266) */
   int precision;     /* 0 plain FP64, 1 mixed */
   int chi_max;       /* reference args.max_bond_dim, 1..256 */
   int reserved;

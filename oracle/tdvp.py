@@ -18,7 +18,7 @@ ONE = np.ones((1, 1, 1), dtype=complex)
 class TDVPOracle:
     def __init__(self, As, Ws, algorithm: str, dt: float, chi_max: int, eps: float,
                  gauge: str = "reference", expm: str = "dense", krylov_tol: float = 1e-14):
-        if algorithm not in ("1tdvp", "2tdvp"):
+        if algorithm not in ("1tdvp", "2tdvp", "a1tdvp"):
             raise ValueError(algorithm)
         if gauge not in ("reference", "consistent"):
             raise ValueError(gauge)
@@ -107,8 +107,43 @@ class TDVPOracle:
             self._sweep_right_2()
             self._sweep_left_2()
         else:
+            if self.algorithm == "a1tdvp":
+                self._new_target_bond_dims()
             self._sweep_right_1()
             self._sweep_left_1()
+
+    def _new_target_bond_dims(self) -> None:
+        """tdvp.py:212-266 with the 'exact_exponential' propagator
+        exp(-i (pi/2) (dt/2) H) (tdvp.py:256-257).  Reference mode replays the
+        reference's order (make_site_canonical before each environment layer,
+        right environments not rebuilt while the centre walks left); consistent
+        mode moves the centre with reduced QRs and refreshes the environment it
+        uses.  Targets of bonds 1..N-1: max(1, min(k, min(max_bond, len(s))))
+        with k the first i with ||s[i:]|| < eps (tdvp.py:261-265)."""
+        n = len(self.A)
+        for site in range(n - 1):
+            if self.gauge == "reference":
+                M.make_site_canonical(self.A, site + 1)
+            else:
+                M.orthonormalize_left(self.A, site)
+            self._update_env("left", site)
+        for bond in range(n - 1, 0, -1):
+            Al, Ar = self.A[bond - 1], self.A[bond]
+            theta = loc.merge_theta(Al, Ar)
+            W2 = loc.merge_w(self.W[bond - 1], self.W[bond])
+            theta = self._evolve(theta, self._get_env("left", bond - 2), self._get_env("right", bond + 1),
+                                 W2, self.dt / 2)
+            chil, chir = Al.shape[1], Ar.shape[2]
+            Mt = theta.reshape(2, 2, chil, chir).transpose(0, 2, 1, 3).reshape(2 * chil, 2 * chir)
+            s = np.linalg.svd(Mt, compute_uv=False)
+            cap = min(self.max_bond[bond], len(s))
+            k = next((i for i in range(len(s)) if np.linalg.norm(s[i:]) < self.eps), cap)
+            self.target[bond] = max(1, min(k, cap))
+            if self.gauge == "reference":
+                M.make_site_canonical(self.A, bond - 1)
+            else:
+                M.orthonormalize_right(self.A, bond)
+                self._update_env("right", bond)
 
     def _sweep_right_2(self) -> None:
         """tdvp.py:107-125"""

@@ -1,4 +1,5 @@
-"""1-site / 2-site TDVP on the GPU (drop-in for reference algorithms/tdvp.py:10-188).
+"""1-site / 2-site / adaptive 1-site TDVP on the GPU (drop-in for reference
+algorithms/tdvp.py:10-266).
 
 ``TDVP(psi_0, H, args)`` takes the reference's MPS and MPO objects and reads the
 same ``args`` fields (algorithm, step_size, max_bond_dim, svd_epsilon) plus the
@@ -13,7 +14,11 @@ Differences from the reference, all deliberate:
     mps.py:110-117 gauge side effect leave them stale; its own 2TDVP/1TDVP do
     not reproduce exact evolution at full bond dimension because of it);
   * the local exponentials are Krylov (FP64 recurrence) instead of dense eigh;
-  * 'a1tdvp' is not provided.
+  * 'a1tdvp' chooses its bond targets with the 'exact_exponential' propagator
+    whatever --approximative-evolution-method says: the reference's default
+    'taylor' needs numpy.math (removed in numpy 2, tdvp.py:247) and its
+    'expm_multiply' branch exponentiates the stored transpose of H_eff
+    (tdvp.py:252-253).
 """
 from __future__ import annotations
 
@@ -38,9 +43,10 @@ class TDVP(Algorithm):
         self._engine = None
         super().__init__(psi_0, H, args)
         algo = getattr(args, "algorithm", "2tdvp")
-        if algo not in ("1tdvp", "2tdvp"):
+        codes = {"1tdvp": 1, "2tdvp": 2, "a1tdvp": 3}
+        if algo not in codes:
             raise NotImplementedError(f"algorithm {algo!r} is not provided by the GPU TDVP")
-        self._algo = 1 if algo == "1tdvp" else 2
+        self._algo = codes[algo]
         prec = _precision_code(args)
         tol = float(getattr(args, "krylov_tol", 0.0) or 0.0)
         if tol <= 0.0:

@@ -55,7 +55,8 @@ Engine::Engine(int device, const EngineParams& p, const std::vector<const c128*>
             "one MPO tensor per cell");
   QCA_CHECK((int)A_host.size() == N_ && (int)chis.size() == N_ + 1, "one MPS tensor per cell");
   QCA_CHECK(chis[0] == 1 && chis[N_] == 1, "open boundary bonds must be 1");
-  QCA_CHECK(p.algorithm == 1 || p.algorithm == 2, "algorithm must be 1 (1TDVP) or 2 (2TDVP)");
+  QCA_CHECK(p.algorithm >= 1 && p.algorithm <= 3,
+            "algorithm must be 1 (1TDVP), 2 (2TDVP) or 3 (adaptive 1TDVP)");
   QCA_CHECK(p.precision == PREC_FP64 || p.precision == PREC_MIXED, "unknown precision");
   QCA_CHECK(p.chi_max >= 1 && p.chi_max <= 256, "max bond dimension must be in 1..256");
   QCA_CHECK(Wdl[0] == 1 && Wdr[N_ - 1] == 1, "MPO boundary bonds must be 1");
@@ -68,8 +69,9 @@ Engine::Engine(int device, const EngineParams& p, const std::vector<const c128*>
   chi_ = chis;
   cap_.resize(N_ + 1);
   target_.resize(N_ + 1);
+  maxdim_.resize(N_ + 1);
   for (int i = 0; i <= N_; ++i) {
-    target_[i] = sat_pow2_min(i, N_ - i, p.chi_max);
+    maxdim_[i] = target_[i] = sat_pow2_min(i, N_ - i, p.chi_max);
     cap_[i] = std::max(target_[i], chi_[i]);
   }
   Dl_ = Wdl;
@@ -85,7 +87,7 @@ Engine::Engine(int device, const EngineParams& p, const std::vector<const c128*>
     Wf_[i].upload(build_sparse(W_host[i], 2, Dl_[i], Dr_[i], false), 2 * Dl_[i]);
     Wb_[i].upload(build_sparse(W_host[i], 2, Dl_[i], Dr_[i], true), 2 * Dr_[i]);
   }
-  if (p.algorithm == 2) {
+  if (p.algorithm != 1) {  // two-site blocks: 2TDVP and the adaptive targets
     W2f_.resize(N_ - 1);
     for (int s = 0; s + 1 < N_; ++s) {
       auto w2 = merge_mpo(W_host[s], W_host[s + 1], Dl_[s], Dr_[s], Dr_[s + 1]);
@@ -539,10 +541,44 @@ void Engine::step() {
     sweep_right_2();
     sweep_left_2();
   } else {
+    if (p_.algorithm == 3) compute_targets();
     sweep_right_1();
     sweep_left_1();
   }
   sync();
+}
+
+// Adaptive 1TDVP bond targets (reference tdvp.py:212-266 with the
+// exact_exponential propagator): walk the centre to the right end with reduced
+// QRs, refreshing the left environments; then from the right end, evolve a
+// copy of the two-site block of every bond by dt/2, take its truncation rank
+// k (first i with ||s[i:]|| < eps, capped by min(max bond, len(s))) as the
+// target max(1, k), and move the centre left with a reduced QR, refreshing the
+// right environment the next block uses.  The state itself is only regauged.
+void Engine::compute_targets() {
+  int cr, cc;
+  for (int site = 0; site + 1 < N_; ++site) {
+    qr_left_site(site, chi_[site + 1], cbond_, cr, cc, false);
+    absorb_left(site + 1, cbond_, cr, cc);
+    update_env(true, site);
+  }
+  for (int bond = N_ - 1; bond >= 1; --bond) {
+    const int s = bond - 1;
+    const int chil = chi_[s], chim = chi_[s + 1], chir = chi_[s + 2];
+    merge_theta(s, chil, chim, chir);
+    evolve(0, s, theta_, 4L * chil * chir, chil, chir, p_.dt / 2);
+    const int rows = 2 * chil, cols = 2 * chir;
+    launch_theta_to_svd(theta_, svdX_, chil, chir, 0, s_);
+    launch_jacobi_svd(svdX_, rows, cols, sigma_, perm_, svdres_, maxdim_[bond], p_.eps,
+                      std::min(rows, cols), s_, jw_);
+    QCA_HIP(hipMemcpyAsync(svdres_host_, svdres_, sizeof(SvdResult), hipMemcpyDeviceToHost, s_));
+    sync();
+    stats_.svd_sweeps += svdres_host_->nsweeps;
+    target_[bond] = std::max(1, svdres_host_->k);
+    qr_right_site(bond, chi_[bond], cbond_, cr, cc, false);
+    absorb_right(bond - 1, cbond_, cr, cc);
+    update_env(false, bond);
+  }
 }
 
 // ---------------------------------------------------------------- measurement

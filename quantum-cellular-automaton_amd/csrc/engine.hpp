@@ -12,7 +12,7 @@ enum Precision { PREC_FP64 = 0, PREC_MIXED = 1 };
 
 struct EngineParams {
   int ncells = 0;
-  int algorithm = 2;       // 1 = 1TDVP, 2 = 2TDVP
+  int algorithm = 2;       // 1 = 1TDVP, 2 = 2TDVP, 3 = adaptive 1TDVP
   int precision = PREC_FP64;
   int chi_max = 32;
   double dt = 0.1;         // time step (propagators use exp(-i pi/2 dt H))
@@ -75,6 +75,7 @@ class Engine {
   void sweep_right_1();
   void sweep_left_1();
   void canonicalize_initial();
+  void compute_targets();
   void qr_left_site(int site, int target, c128* Cout, int& crow, int& ccol, bool complete);
   void qr_right_site(int site, int target, c128* Cout, int& crow, int& ccol, bool complete);
   void absorb_left(int site, const c128* C, int crow, int ccol);   // A[site] = C * A[site]
@@ -87,7 +88,7 @@ class Engine {
   int device_;
   hipStream_t s_ = nullptr;
   bool own_stream_ = false;
-  std::vector<int> chi_, cap_, target_;
+  std::vector<int> chi_, cap_, target_, maxdim_;
   std::vector<int> Dl_, Dr_;
   int Dmax_ = 1;
   // MPO (forward/backward sparse forms per site, merged pairs for 2TDVP)

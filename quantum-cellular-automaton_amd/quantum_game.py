@@ -57,7 +57,7 @@ class QuantumGame(object):
         if args.algorithm == "exact":
             local.step_size = args.step_size * args.plot_step_interval
             algorithm = Exact(psi_0=initial_state, H=self.H, args=local)
-        elif args.algorithm in ("1tdvp", "2tdvp"):
+        elif args.algorithm in ("1tdvp", "2tdvp", "a1tdvp"):
             algorithm = TDVP(psi_0=initial_state, H=self.H, args=local)
         else:
             raise NotImplementedError(f"algorithm {args.algorithm!r} is not provided")

@@ -300,8 +300,35 @@ def g_padded():
     _save("padded_runs", **out)
 
 
+def g_a1tdvp():
+    """Adaptive 1TDVP (a1tdvp) with --approximative-evolution-method
+    exact_exponential (the default 'taylor' needs numpy.math, removed in numpy 2;
+    tdvp.py:247), from product states and from a bond-padded state, measured
+    every step."""
+    ARGS.approximative_evolution_method = "exact_exponential"
+    out = {}
+    cases = [
+        (10, 1, 1, 2, single_plist(10), 0.1, 4, 16, 1e-10, None),
+        (11, 2, 2, 4, [0, 0, 0, 1, 1, 0, 1, 0, 0, 0, 0], 0.1, 3, 8, 1e-6, None),
+        (9, 1, 1, 3, [0.3, 0, 1, 0.5, 0, 0, 1, 0.1, 0], 0.05, 3, 8, 1e-8, None),
+        (10, 1, 1, 2, [0, 1, 0, 0, 1, 0, 0, 0, 1, 0], 0.1, 4, 8, 1e-6, 21),
+    ]
+    for i, (n, d, lo, hi, pl, dt, steps, chi, eps, seed) in enumerate(cases):
+        As = padded_tensors(pl, chi, 1e-3, seed) if seed is not None else None
+        r = run_reference("a1tdvp", n, d, lo, hi, pl, dt, steps, chi, eps, 1, tensors=As)
+        out[f"c{i}_params"] = np.array([n, d, lo, hi, dt, steps, chi, eps, 1], dtype=float)
+        out[f"c{i}_plist"] = np.array(pl, dtype=float)
+        out[f"c{i}_padded"] = np.array(seed is not None)
+        if As is not None:
+            for j, A in enumerate(As):
+                out[f"c{i}_A{j}"] = A
+        for k in ("population", "d_population", "single_site_entropy", "bond_dims"):
+            out[f"c{i}_{k}"] = r[k]
+    _save("a1tdvp_runs", **out)
+
+
 ALL = dict(padded=g_padded, mpo=g_mpo, exact=g_exact, fullchi=g_fullchi, units=g_units, short=g_short,
-           config1=g_config1)
+           config1=g_config1, a1tdvp=g_a1tdvp)
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()

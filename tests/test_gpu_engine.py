@@ -39,7 +39,12 @@ CASES = [
     ("1tdvp", 10, 1, range(1, 2), "padded", 0.1, 8, 8, 0.0),
     ("1tdvp", 11, 2, range(2, 4), "padded", 0.05, 6, 16, 0.0),
     ("1tdvp", 9, 2, range(2, 4), "triple", 0.1, 6, 16, 0.0),
+    ("a1tdvp", 10, 1, range(1, 2), "single", 0.1, 6, 16, 1e-10),
+    ("a1tdvp", 11, 2, range(2, 4), "padded", 0.05, 5, 8, 1e-6),
+    ("a1tdvp", 12, 2, range(2, 5), "triple", 0.1, 5, 16, 1e-8),
 ]
+
+ALG_CODE = {"1tdvp": 1, "2tdvp": 2, "a1tdvp": 3}
 
 
 def initial(kind, n, chi, seed=0):
@@ -78,7 +83,7 @@ def test_engine_matches_oracle(gpu, case):
     Ws = AU.mpo_tensors(n, d, iv)
     o = OT.TDVPOracle([a.copy() for a in As], Ws, alg, dt, chi, eps, gauge="consistent", expm="lanczos")
     ref = OT.run_trajectory(o, steps, 1)
-    eng = make_engine([a.copy() for a in As], Ws, 1 if alg == "1tdvp" else 2, chi, dt, eps)
+    eng = make_engine([a.copy() for a in As], Ws, ALG_CODE[alg], chi, dt, eps)
     pops, bds = gpu_trajectory(eng, steps)
     assert np.array_equal(bds, ref["bond_dims"].astype(int))
     # Tolerance 1e-10, or ten times the trajectory's own conditioning when that
@@ -247,3 +252,38 @@ def test_batch_step_equals_sequential(gpu):
     for _ in range(3):
         o.do_time_step()
     assert np.abs(e.rho()[:, 1, 1].real - OM.measure_consistent(o.A)[0]).max() < 1e-10
+
+
+def test_server_on_gpu_matches_engines_run_alone(gpu):
+    """Served requests (continuous batching over 3 workers, one engine and
+    stream per request) report the same <n_i> every 10 steps as the same
+    instances stepped alone."""
+    import serve
+    import workloads
+    arrivals = serve.poisson_arrivals(rate=50.0, window_s=0.12, seed=5, big_fraction=0.0, max_steps=25)
+    assert 2 <= len(arrivals) <= 12
+    make = serve.gpu_engine_factory(precision=0, chi_cap=16)
+    out = serve.Server(make, workers=3).run(arrivals)
+    assert out["requests"] == len(arrivals) and out["p99_latency_s"] >= out["p50_latency_s"]
+    for _, r in arrivals:
+        e = make(r.inst)
+        for k, rep in enumerate(r.reports):
+            e.step(min(10, r.steps - 10 * k))
+            assert np.array_equal(e.rho()[:, 1, 1].real, rep)
+        e.close()
+
+
+def test_a1tdvp_first_steps_match_reference_golden(gpu):
+    """Adaptive 1TDVP: the engine takes the reference's first step (chosen
+    bond targets and <n_i>) on every golden case."""
+    g = golden("a1tdvp_runs")
+    for c in range(4):
+        n, d, lo, hi, dt, steps, chi, eps, itv = g[f"c{c}_params"]
+        n = int(n)
+        As = ([g[f"c{c}_A{j}"] for j in range(n)] if bool(g[f"c{c}_padded"])
+              else OM.product_state(g[f"c{c}_plist"]))
+        eng = make_engine([a.copy() for a in As], AU.mpo_tensors(n, int(d), range(int(lo), int(hi))), 3,
+                          int(chi), dt, eps)
+        pops, bds = gpu_trajectory(eng, 2)
+        assert np.array_equal(bds, g[f"c{c}_bond_dims"][:2].astype(int))
+        assert np.abs(pops - g[f"c{c}_population"][:2]).max() < 1e-9

@@ -227,6 +227,41 @@ def test_reference_mode_replays_whole_trajectory():
     assert np.abs(r["population"] - f["1tdvp_itv10_population"]).max() < 1e-12
 
 
+def a1_case(c):
+    g = golden("a1tdvp_runs")
+    n, d, lo, hi, dt, steps, chi, eps, itv = g[f"c{c}_params"]
+    n = int(n)
+    As = ([g[f"c{c}_A{j}"] for j in range(n)] if bool(g[f"c{c}_padded"])
+          else OM.product_state(g[f"c{c}_plist"]))
+    return g, As, n, int(d), range(int(lo), int(hi)), dt, int(steps), int(chi), eps
+
+
+@pytest.mark.parametrize("c", [0, 1, 2, 3])
+def test_a1tdvp_reference_mode_replays_reference(c):
+    """Adaptive 1TDVP (tdvp.py:212-266, exact_exponential propagator): the
+    reference-mode replay reproduces the reference's trajectories, including
+    the bond targets it chooses at every step."""
+    g, As, n, d, iv, dt, steps, chi, eps = a1_case(c)
+    o = OT.TDVPOracle([a.copy() for a in As], AU.mpo_tensors(n, d, iv), "a1tdvp", dt, chi, eps,
+                      gauge="reference", expm="lanczos")
+    r = OT.run_trajectory(o, steps, 1)
+    assert np.abs(r["population"] - g[f"c{c}_population"]).max() < 1e-9
+    assert np.array_equal(r["bond_dims"], g[f"c{c}_bond_dims"])
+
+
+@pytest.mark.parametrize("c", [0, 1, 2, 3])
+def test_a1tdvp_consistent_first_step(c):
+    """The gauge-consistent a1tdvp takes the reference's first step (targets
+    and state) and its later steps differ only by the reference's stale
+    environments."""
+    g, As, n, d, iv, dt, steps, chi, eps = a1_case(c)
+    o = OT.TDVPOracle([a.copy() for a in As], AU.mpo_tensors(n, d, iv), "a1tdvp", dt, chi, eps,
+                      gauge="consistent", expm="lanczos")
+    r = OT.run_trajectory(o, 2, 1)
+    assert np.abs(r["population"] - g[f"c{c}_population"][:2]).max() < 1e-9
+    assert np.array_equal(r["bond_dims"], g[f"c{c}_bond_dims"][:2])
+
+
 def test_reference_trajectory_is_not_exact():
     """The reference's own TDVP at full bond dimension deviates from its Exact
     (stale right environments after re-canonicalisation and measurement)."""
@@ -243,7 +278,7 @@ def full_rank_state(n, seed):
     return As
 
 
-@pytest.mark.parametrize("alg", ["2tdvp", "1tdvp"])
+@pytest.mark.parametrize("alg", ["2tdvp", "1tdvp", "a1tdvp"])
 @pytest.mark.parametrize("expm", ["dense", "lanczos"])
 def test_consistent_full_rank_is_exact(alg, expm):
     """Projector-splitting TDVP at full bond dimension on a full-rank state is
