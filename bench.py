@@ -146,6 +146,26 @@ def batch_bench(torch, a):
             "speedup_over_sequential": rates["batched"] / rates["sequential"]}
 
 
+def serve_bench(a):
+    """A served window: Poisson arrivals at `a.serve` requests/s for
+    `a.serve_window` s (default_rng(5)), each request a batched-workload draw
+    (or, with probability `a.serve_big_fraction`, the 101-cell chi-256 case),
+    steps capped at `a.serve_max_steps` and chi at `a.serve_chi_cap`; <n_i>
+    returned every 10 steps; `a.batch_threads` workers; mixed precision."""
+    import serve
+    arrivals = serve.poisson_arrivals(a.serve, a.serve_window, seed=5, big_fraction=a.serve_big_fraction,
+                                      max_steps=a.serve_max_steps)
+    log(f"served window: {len(arrivals)} requests over {a.serve_window:.0f} s")
+    srv = serve.Server(serve.gpu_engine_factory(precision=1, chi_cap=a.serve_chi_cap), workers=a.batch_threads)
+    out = srv.run(arrivals)
+    out.update({"arrival_rate_per_s": a.serve, "window_s": a.serve_window, "max_steps": a.serve_max_steps,
+                "chi_cap": a.serve_chi_cap, "big_fraction": a.serve_big_fraction, "workers": a.batch_threads,
+                "report_every_steps": 10})
+    log(f"served window: p50 {out['p50_latency_s']:.2f} s p99 {out['p99_latency_s']:.2f} s, "
+        f"{out['site_updates_per_s']:.0f} site-updates/s")
+    return out
+
+
 def cpu_baseline(As, seconds: float):
     """Oracle 2TDVP site-updates on the host at the benchmark state (bounded)."""
     from oracle import automaton as AU, tdvp as OT
@@ -187,6 +207,13 @@ def main() -> int:
     ap.add_argument("--batch-threads", type=int, default=16)
     ap.add_argument("--batch-warmup", type=int, default=10)
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--serve", type=float, default=0.0,
+                    help="also run a served window at this Poisson arrival rate (requests/s; "
+                         "default 0: skip)")
+    ap.add_argument("--serve-window", type=float, default=10.0)
+    ap.add_argument("--serve-max-steps", type=int, default=50)
+    ap.add_argument("--serve-chi-cap", type=int, default=32)
+    ap.add_argument("--serve-big-fraction", type=float, default=0.0)
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the H_eff micro-benchmark and print its roofline object "
                          "(the command the kernel profiles are taken with)")
@@ -195,7 +222,7 @@ def main() -> int:
         print("bench.py measures one GPU: every box in this pool has a single MI355X, and the "
               "2TDVP sweep of one instance does not shard; run with --gpus 1", file=sys.stderr)
         return 2
-    if a.batch:
+    if a.batch or a.serve:
         # one hardware queue per worker stream (HIP's default is 4); must be set
         # before the HIP runtime starts
         os.environ["GPU_MAX_HW_QUEUES"] = str(min(16, max(4, a.batch_threads)))
@@ -261,6 +288,7 @@ def main() -> int:
     ach32 = fl / (ms32 * 1e-3) / 1e12
     ach64 = fl / (ms64 * 1e-3) / 1e12
     batched = batch_bench(torch, a) if a.batch else None
+    served = serve_bench(a) if a.serve else None
     cpu = None
     if not a.no_cpu_baseline:
         log(f"cpu baseline ({a.cpu_seconds:.0f} s budget)")
@@ -302,6 +330,7 @@ def main() -> int:
                               "traffic": HEFF2_TRAFFIC_BYTES["fp64"]}},
         "cpu_baseline": cpu,
         "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
     return 0

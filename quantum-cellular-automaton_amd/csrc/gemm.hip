@@ -8,14 +8,16 @@
 // 32x32 block = 2x2 MFMA 16x16 tiles.  Operands are read with generic two-level
 // strides (see GemmDesc) so that every TDVP contraction is a single launch with
 // no permutation copies; the load loop walks the unit-stride index so global
-// reads stay coalesced.  Launches with too few output tiles to fill 256 CUs
-// split K and reduce the partial tiles in a second, deterministic pass.
+// reads stay coalesced; the next K-tile is loaded into registers while the
+// current one is multiplied.  Launches with too few output tiles to fill 256
+// CUs split K and reduce the partial tiles in a second, deterministic pass.
 #include "qca_common.hpp"
 
 namespace qca {
 namespace {
 
 constexpr int BM = 64, BN = 64, BK = 16, PAD = 1;
+constexpr int LOADS = BM * BK / 256;  // operand elements per thread per K-tile
 
 template <typename T>
 struct Mfma;
@@ -82,47 +84,68 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmDesc d, const cplx<T>* __
   const T sa = d.conj_a ? T(-1) : T(1);
   const T sb = d.conj_b ? T(-1) : T(1);
 
-  for (int k0 = k_begin; k0 < k_end; k0 += BK) {
-    // ---- stage A (BM x BK) and B (BK x BN) into LDS, split into re/im planes
+  // Operand elements of one K-tile, loaded into registers one tile ahead so
+  // that the global loads of tile k0+BK overlap the MFMAs of tile k0.
+  const bool flatK = d.K2 >= d.K;  // single-level K: no k1/k2 split
+  cplx<T> pa[LOADS], pb[LOADS];
+  auto load_tile = [&](int k0) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      int ml, kl;
-      if (A_MCONT) {
-        ml = tid & 63;
-        kl = (tid >> 6) + 4 * r;
-      } else {
-        kl = tid & 15;
-        ml = (tid >> 4) + 16 * r;
-      }
+    for (int r = 0; r < LOADS; ++r) {
+      const int e = tid + 256 * r;
+      const int ml = A_MCONT ? (e % BM) : (e / BK);
+      const int kl = A_MCONT ? (e / BM) : (e % BK);
       const int m = m0 + ml, k = k0 + kl;
       cplx<T> v{T(0), T(0)};
       if (m < d.M && k < k_end) {
-        const int k1 = k / d.K2, k2 = k - k1 * d.K2;
-        v = Ab[m * d.a_m + k1 * d.a_k1 + k2 * d.a_k2];
+        if (flatK) {
+          v = Ab[m * d.a_m + k * d.a_k2];
+        } else {
+          const int k1 = k / d.K2, k2 = k - k1 * d.K2;
+          v = Ab[m * d.a_m + k1 * d.a_k1 + k2 * d.a_k2];
+        }
       }
-      As_re[kl][ml] = v.re;
-      As_im[kl][ml] = sa * v.im;
+      pa[r] = v;
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      int nl, kl;
-      if (B_NCONT) {
-        nl = tid & 63;
-        kl = (tid >> 6) + 4 * r;
-      } else {
-        kl = tid & 15;
-        nl = (tid >> 4) + 16 * r;
-      }
+    for (int r = 0; r < LOADS; ++r) {
+      const int e = tid + 256 * r;
+      const int nl = B_NCONT ? (e % BN) : (e / BK);
+      const int kl = B_NCONT ? (e / BN) : (e % BK);
       const int n = n0 + nl, k = k0 + kl;
       cplx<T> v{T(0), T(0)};
       if (n < d.N && k < k_end) {
-        const int k1 = k / d.K2, k2 = k - k1 * d.K2;
-        v = Bb[k1 * d.b_k1 + k2 * d.b_k2 + n * d.b_n];
+        if (flatK) {
+          v = Bb[k * d.b_k2 + n * d.b_n];
+        } else {
+          const int k1 = k / d.K2, k2 = k - k1 * d.K2;
+          v = Bb[k1 * d.b_k1 + k2 * d.b_k2 + n * d.b_n];
+        }
       }
-      Bs_re[kl][nl] = v.re;
-      Bs_im[kl][nl] = sb * v.im;
+      pb[r] = v;
+    }
+  };
+
+  if (k_begin < k_end) load_tile(k_begin);
+  for (int k0 = k_begin; k0 < k_end; k0 += BK) {
+    // ---- registers -> LDS (re/im planes), then prefetch the next tile
+#pragma unroll
+    for (int r = 0; r < LOADS; ++r) {
+      const int e = tid + 256 * r;
+      const int ml = A_MCONT ? (e % BM) : (e / BK);
+      const int kl = A_MCONT ? (e / BM) : (e % BK);
+      As_re[kl][ml] = pa[r].re;
+      As_im[kl][ml] = sa * pa[r].im;
+    }
+#pragma unroll
+    for (int r = 0; r < LOADS; ++r) {
+      const int e = tid + 256 * r;
+      const int nl = B_NCONT ? (e % BN) : (e / BK);
+      const int kl = B_NCONT ? (e / BN) : (e % BK);
+      Bs_re[kl][nl] = pb[r].re;
+      Bs_im[kl][nl] = sb * pb[r].im;
     }
     __syncthreads();
+    if (k0 + BK < k_end) load_tile(k0 + BK);
 
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
@@ -141,8 +164,13 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmDesc d, const cplx<T>* __
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
           acc_re[i][j] = Mfma<T>::mac(ar[i], br[j], acc_re[i][j]);
-          acc_re[i][j] = Mfma<T>::mac(-ai[i], bi[j], acc_re[i][j]);
           acc_im[i][j] = Mfma<T>::mac(ar[i], bi[j], acc_im[i][j]);
+        }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          acc_re[i][j] = Mfma<T>::mac(-ai[i], bi[j], acc_re[i][j]);
           acc_im[i][j] = Mfma<T>::mac(ai[i], br[j], acc_im[i][j]);
         }
     }

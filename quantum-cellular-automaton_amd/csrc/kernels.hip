@@ -50,49 +50,47 @@ __device__ double block_sum(double v, double* sh) {
 
 // ------------------------------------------------------------------ wmix
 // U[q][xc][v] = sum_{i} Wsp[(q,v), i] * Tin[i][xc]
-// Input channels are staged through LDS so every input element is read from
-// memory once; the output rows [q][xc0..xc0+63][0..Dout) are written as one
-// contiguous run.
+// A block owns 64 consecutive xc and runs WMIX_WAVES waves: the input channels
+// of those xc are staged through LDS once (each element read from memory
+// once), the waves share the nq*Dout output channels, and the output rows
+// [q][xc0..xc0+63][0..Dout) are written as contiguous runs.
 constexpr int WMIX_XC = 64;
+constexpr int WMIX_WAVES = 8;
 constexpr int WMIX_DMAX = 16;
 
 template <typename T>
-__global__ __launch_bounds__(64) void wmix_kernel(const cplx<T>* __restrict__ Tin, long in_stride,
-                                                  long nxc, int nin, cplx<T>* __restrict__ U,
-                                                  int nq, int Dout, const int* __restrict__ rowptr,
-                                                  const int* __restrict__ col,
-                                                  const cplx<T>* __restrict__ val,
-                                                  const int* __restrict__ skip) {
+__global__ __launch_bounds__(WMIX_XC * WMIX_WAVES) void wmix_kernel(
+    const cplx<T>* __restrict__ Tin, long in_stride, long nxc, int nin, cplx<T>* __restrict__ U,
+    int nq, int Dout, const int* __restrict__ rowptr, const int* __restrict__ col,
+    const cplx<T>* __restrict__ val, const int* __restrict__ skip) {
   if (skip && *skip) return;
   extern __shared__ unsigned char smem_raw[];
   cplx<T>* sin = reinterpret_cast<cplx<T>*>(smem_raw);        // [nin][64]
-  cplx<T>* sout = sin + (long)nin * WMIX_XC;                   // [64][Dout]
-  const int t = threadIdx.x;
+  cplx<T>* sout = sin + (long)nin * WMIX_XC;                   // [nq][64][Dout]
+  const int t = threadIdx.x & (WMIX_XC - 1), wv = threadIdx.x / WMIX_XC;
   const long xc0 = (long)blockIdx.x * WMIX_XC;
   const long xc = xc0 + t;
   const bool live = xc < nxc;
-  for (int i = 0; i < nin; ++i)
+  for (int i = wv; i < nin; i += WMIX_WAVES)
     sin[i * WMIX_XC + t] = live ? Tin[(long)i * in_stride + xc] : cplx<T>{T(0), T(0)};
+  __syncthreads();
+  for (int o = wv; o < nq * Dout; o += WMIX_WAVES) {
+    const int q = o / Dout, v = o - q * Dout;
+    T ar = 0, ai = 0;
+    for (int z = rowptr[o]; z < rowptr[o + 1]; ++z) {
+      const cplx<T> x = sin[col[z] * WMIX_XC + t];
+      const cplx<T> w = val[z];
+      ar += w.re * x.re - w.im * x.im;
+      ai += w.re * x.im + w.im * x.re;
+    }
+    sout[((long)q * WMIX_XC + t) * Dout + v] = cplx<T>{ar, ai};
+  }
+  __syncthreads();
   const int nrun = (int)min<long>(WMIX_XC, nxc - xc0);
   for (int q = 0; q < nq; ++q) {
-    __syncthreads();
-#pragma unroll
-    for (int v = 0; v < WMIX_DMAX; ++v) {
-      if (v < Dout) {
-        const int o = q * Dout + v;
-        T ar = 0, ai = 0;
-        for (int z = rowptr[o]; z < rowptr[o + 1]; ++z) {
-          const cplx<T> x = sin[col[z] * WMIX_XC + t];
-          const cplx<T> w = val[z];
-          ar += w.re * x.re - w.im * x.im;
-          ai += w.re * x.im + w.im * x.re;
-        }
-        sout[t * Dout + v] = cplx<T>{ar, ai};
-      }
-    }
-    __syncthreads();
     cplx<T>* dst = U + ((long)q * nxc + xc0) * Dout;
-    for (int e = t; e < nrun * Dout; e += WMIX_XC) dst[e] = sout[e];
+    const cplx<T>* src = sout + (long)q * WMIX_XC * Dout;
+    for (int e = threadIdx.x; e < nrun * Dout; e += WMIX_XC * WMIX_WAVES) dst[e] = src[e];
   }
 }
 
@@ -559,9 +557,10 @@ void launch_wmix(const cplx<T>* Tin, long in_stride, long nxc, int nin, cplx<T>*
   QCA_CHECK(Dout <= WMIX_DMAX, "MPO bond dimension above 16");
   QCA_CHECK(nin <= 64, "too many input channels in MPO mixing");
   if (nxc == 0) return;
-  const size_t shm = ((size_t)nin * WMIX_XC + (size_t)WMIX_XC * Dout) * sizeof(cplx<T>);
+  const size_t shm = ((size_t)nin * WMIX_XC + (size_t)nq * WMIX_XC * Dout) * sizeof(cplx<T>);
+  QCA_CHECK(shm <= 150 * 1024, "MPO mixing stage exceeds LDS");
   ensure_dynamic_lds((const void*)wmix_kernel<T>, shm);
-  hipLaunchKernelGGL(wmix_kernel<T>, dim3(ceil_div(nxc, WMIX_XC)), dim3(WMIX_XC), shm, s, Tin,
+  hipLaunchKernelGGL(wmix_kernel<T>, dim3(ceil_div(nxc, WMIX_XC)), dim3(WMIX_XC * WMIX_WAVES), shm, s, Tin,
                      in_stride, nxc, nin, U, nq, Dout, w.rowptr, w.col, w.val, skip);
   QCA_HIP(hipGetLastError());
 }
