@@ -61,21 +61,20 @@ struct JacobiGeom {
   int rows, cols, bs, nb, kspl;
 };
 
-// Global column of local column c of block pair p in round r.
-__device__ __forceinline__ int pair_col(const JacobiGeom& g, int p, int round, int c) {
-  const int bi = rr_player(p, round, g.nb), bj = rr_player(g.nb - 1 - p, round, g.nb);
-  return c < g.bs ? bi * g.bs + c : bj * g.bs + (c - g.bs);
-}
-
 // Stages rows [r0, r0+nr) of the 2bs columns of pair p into a row-major LDS
-// tile with a padded row stride (JAC_LMAX + 1) that spreads the banks.
+// tile with a padded row stride (JAC_LMAX + 1) that spreads the banks.  Lane
+// bits give the row, the wave index walks the columns: no divisions, and each
+// wave reads one contiguous 64-row run of a column.
 __device__ void stage_tile(const c128* __restrict__ X, const JacobiGeom& g, int p, int round,
                            int r0, int nr, c128* tile) {
+  static_assert(JAC_RC == 64, "one row per lane");
   const int L = 2 * g.bs;
-  for (int e = threadIdx.x; e < L * nr; e += JAC_NT) {
-    const int c = e / nr, r = e - c * nr;
-    const int gc = pair_col(g, p, round, c);
-    tile[r * (JAC_LMAX + 1) + c] = gc < g.cols ? X[(long)gc * g.rows + r0 + r] : c128{0.0, 0.0};
+  const int bi = rr_player(p, round, g.nb), bj = rr_player(g.nb - 1 - p, round, g.nb);
+  const int r = threadIdx.x & 63;
+  for (int c = threadIdx.x >> 6; c < L; c += JAC_NT / 64) {
+    const int gc = c < g.bs ? bi * g.bs + c : bj * g.bs + (c - g.bs);
+    tile[r * (JAC_LMAX + 1) + c] =
+        (gc < g.cols && r < nr) ? X[(long)gc * g.rows + r0 + r] : c128{0.0, 0.0};
   }
 }
 
@@ -269,18 +268,20 @@ __global__ __launch_bounds__(JAC_NT) void jac_apply(c128* __restrict__ X, Jacobi
   const c128* jp = Jin + (long)p * L * L;
   for (int e = threadIdx.x; e < L * L; e += JAC_NT) J[e] = jp[e];
   __syncthreads();
-  for (int e = threadIdx.x; e < L * nr; e += JAC_NT) {
-    const int c = e / nr, r = e - c * nr;
-    const int gc = pair_col(g, p, round, c);
-    if (gc >= g.cols) continue;
-    double re = 0, im = 0;
-    for (int k = 0; k < L; ++k) {
-      const c128 x = tile[r * (JAC_LMAX + 1) + k], j = J[c * L + k];
-      re += x.re * j.re - x.im * j.im;
-      im += x.re * j.im + x.im * j.re;
+  const int bi = rr_player(p, round, g.nb), bj = rr_player(g.nb - 1 - p, round, g.nb);
+  const int r = threadIdx.x & 63;
+  if (r < nr)
+    for (int c = threadIdx.x >> 6; c < L; c += JAC_NT / 64) {
+      const int gc = c < g.bs ? bi * g.bs + c : bj * g.bs + (c - g.bs);
+      if (gc >= g.cols) continue;
+      double re = 0, im = 0;
+      for (int k = 0; k < L; ++k) {
+        const c128 x = tile[r * (JAC_LMAX + 1) + k], j = J[c * L + k];
+        re += x.re * j.re - x.im * j.im;
+        im += x.re * j.im + x.im * j.re;
+      }
+      X[(long)gc * g.rows + r0 + r] = c128{re, im};
     }
-    X[(long)gc * g.rows + r0 + r] = c128{re, im};
-  }
 }
 
 // Closes a sweep: converged when it made no rotation.
@@ -430,17 +431,17 @@ __global__ __launch_bounds__(JAC_NT) void jac_small(c128* __restrict__ X, int ro
       const int nr = min(JAC_RC, rows - r0);
       stage_tile(X, g, 0, 0, r0, nr, tile);
       __syncthreads();
-      for (int e = tid; e < L * nr; e += JAC_NT) {
-        const int c = e / nr, r = e - c * nr;
-        if (c >= cols) continue;
-        double re = 0, im = 0;
-        for (int k = 0; k < L; ++k) {
-          const c128 x = tile[r * (JAC_LMAX + 1) + k], jj = J[c * L + k];
-          re += x.re * jj.re - x.im * jj.im;
-          im += x.re * jj.im + x.im * jj.re;
+      const int r = tid & 63;
+      if (r < nr)
+        for (int c = tid >> 6; c < cols; c += JAC_NT / 64) {
+          double re = 0, im = 0;
+          for (int k = 0; k < L; ++k) {
+            const c128 x = tile[r * (JAC_LMAX + 1) + k], jj = J[c * L + k];
+            re += x.re * jj.re - x.im * jj.im;
+            im += x.re * jj.im + x.im * jj.re;
+          }
+          X[(long)c * rows + r0 + r] = c128{re, im};
         }
-        X[(long)c * rows + r0 + r] = c128{re, im};
-      }
       __syncthreads();
     }
   }
