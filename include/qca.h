@@ -91,6 +91,14 @@ int qca_engine_step(qca_engine* e, int nsteps);
 /* Single-site reduced density matrices, host out[8*N] = rho_i[p][q] as complex.
  * <n_i> = Re rho_i[1][1] (reference mps.py:122-132). */
 int qca_engine_measure(qca_engine* e, double* rho_out);
+/* The four rows reference Algorithm.measure writes (algorithm.py:58-63 ->
+ * mps.py:100-140), computed on the device: population[N] = Re rho_i[1][1],
+ * d_population[N] = its rounding (half to even), single_site_entropy[N] =
+ * -Tr rho_i log2 rho_i (eigenvalues of the 2x2 Hermitian part, 0 log 0 = 0),
+ * bond_dims[N+1] = current bond dimensions (recorded before any gauge change,
+ * mps.py:106-109).  All host arrays of doubles. */
+int qca_engine_observables(qca_engine* e, double* population, double* d_population,
+                           double* single_site_entropy, double* bond_dims);
 /* Current bond dimensions, host out[N+1] (reference mps.py:106-109). */
 int qca_engine_bond_dims(qca_engine* e, int* out);
 /* Copy site tensor i to host complex128 (2, chi_i, chi_{i+1}). */

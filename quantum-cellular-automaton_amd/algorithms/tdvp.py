@@ -26,7 +26,7 @@ import numpy as np
 
 from qca_native import Engine
 
-from .algorithm import Algorithm, write_observables
+from .algorithm import Algorithm
 
 
 def _precision_code(args) -> int:
@@ -87,9 +87,9 @@ class TDVP(Algorithm):
         self._host_fresh = False
 
     def measure(self, population, d_population, single_site_entropy, bond_dims) -> None:
-        rho = self._engine.rho()
-        write_observables(rho, self._engine.bond_dims(), population, d_population,
-                          single_site_entropy, bond_dims)
+        """Reference algorithm.py:58-63: the four rows are computed on the
+        device (qca_engine_observables) without moving the centre."""
+        self._engine.observables(population, d_population, single_site_entropy, bond_dims)
 
     def populations(self) -> np.ndarray:
         return self._engine.rho()[:, 1, 1].real.copy()

@@ -299,6 +299,18 @@ int qca_engine_measure(qca_engine* e, double* rho_out) {
   });
 }
 
+int qca_engine_observables(qca_engine* e, double* population, double* d_population,
+                           double* single_site_entropy, double* bond_dims) {
+  return guarded([&] {
+    QCA_CHECK(e && e->eng && population && d_population && single_site_entropy && bond_dims,
+              "null argument");
+    QCA_HIP(hipSetDevice(e->eng->device()));
+    const int n = e->eng->ncells();
+    for (int i = 0; i <= n; ++i) bond_dims[i] = e->eng->chi(i);
+    e->eng->observables(population, d_population, single_site_entropy);
+  });
+}
+
 int qca_engine_bond_dims(qca_engine* e, int* out) {
   return guarded([&] {
     QCA_CHECK(e && e->eng && out, "null argument");

@@ -173,6 +173,7 @@ Engine::Engine(int device, const EngineParams& p, const std::vector<const c128*>
   measE_ = dmalloc<c128>(cmax * cmax);
   measT_ = dmalloc<c128>(2 * cmax * cmax);
   rho_ = dmalloc<c128>(4L * N_);
+  obs_ = dmalloc<double>(3L * N_);
 
   // ---- TDVP construction (reference tdvp.py:20-39): canonicalise, then the
   // right environments of every site.
@@ -586,6 +587,23 @@ void Engine::compute_targets() {
 // E_i = sum_p A_i^T E_{i-1} conj(A_i); equals the reference's centre-site rho
 // (mps.py:100-140) because sites 1..N-1 are right-isometric.
 void Engine::measure(c128* rho_host) {
+  measure_device();
+  QCA_HIP(hipMemcpyAsync(rho_host, rho_, sizeof(c128) * 4L * N_, hipMemcpyDeviceToHost, s_));
+  sync();
+}
+
+void Engine::observables(double* population, double* d_population, double* entropy) {
+  measure_device();
+  launch_observables(rho_, N_, obs_, s_);
+  std::vector<double> h(3L * N_);
+  QCA_HIP(hipMemcpyAsync(h.data(), obs_, sizeof(double) * 3L * N_, hipMemcpyDeviceToHost, s_));
+  sync();
+  std::copy(h.begin(), h.begin() + N_, population);
+  std::copy(h.begin() + N_, h.begin() + 2L * N_, d_population);
+  std::copy(h.begin() + 2L * N_, h.end(), entropy);
+}
+
+void Engine::measure_device() {
   c128 one{1.0, 0.0};
   QCA_HIP(hipMemcpyAsync(measE_, &one, sizeof(one), hipMemcpyHostToDevice, s_));
   for (int site = 0; site < N_; ++site) {
@@ -608,8 +626,6 @@ void Engine::measure(c128* rho_host) {
       gemm<double>(h, A_[site], measT_, measE_);
     }
   }
-  QCA_HIP(hipMemcpyAsync(rho_host, rho_, sizeof(c128) * 4L * N_, hipMemcpyDeviceToHost, s_));
-  sync();
 }
 
 void Engine::get_site(int i, c128* host) const {

@@ -45,6 +45,8 @@ class Engine {
 
   void step();                                        // one time step (both half sweeps)
   void measure(c128* rho_host);                       // 4 complex per site
+  // population, rounded population, single-site entropy (N each, host)
+  void observables(double* population, double* d_population, double* entropy);
   int chi(int bond) const { return chi_[bond]; }
   int ncells() const { return N_; }
   int device() const { return device_; }
@@ -76,6 +78,7 @@ class Engine {
   void sweep_left_1();
   void canonicalize_initial();
   void compute_targets();
+  void measure_device();                               // rho_ <- per-site rho
   void qr_left_site(int site, int target, c128* Cout, int& crow, int& ccol, bool complete);
   void qr_right_site(int site, int target, c128* Cout, int& crow, int& ccol, bool complete);
   void absorb_left(int site, const c128* C, int crow, int ccol);   // A[site] = C * A[site]
@@ -116,6 +119,7 @@ class Engine {
   LanczosCtrl* ctrl_host_ = nullptr;
   LanczosWork lw_;
   c128 *measE_ = nullptr, *measT_ = nullptr, *rho_ = nullptr;
+  double* obs_ = nullptr;
   int kpred_[3] = {8, 8, 8};
   StepStats stats_;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;  // phase timers

@@ -211,7 +211,7 @@ __global__ void qr_gather_right(const c128* __restrict__ Q, const c128* __restri
 
 // ------------------------------------------------------------------ lanczos
 constexpr int LZ_BLOCK = 256;
-constexpr int LZ_EPT = 4;                     // vector elements per thread
+constexpr int LZ_EPT = 2;                     // vector elements per thread
 constexpr int LZ_ELEMS = LZ_BLOCK * LZ_EPT;   // elements per block in the partial reductions
 
 __global__ void lz_norm_partials(const c128* __restrict__ x, long n, double* __restrict__ part) {
@@ -244,23 +244,15 @@ __global__ void lz_scale_first(const c128* __restrict__ x, c128* __restrict__ v0
     v0[i] = c128{x[i].re * inv, x[i].im * inv};
 }
 
-// Partial projections c_i = <V_i, w> over this block's chunk.  Each thread
-// keeps its LZ_EPT elements of w in registers across the i loop.
-template <typename TW>
-__global__ __launch_bounds__(LZ_BLOCK) void lz_project(const LanczosCtrl* ctrl,
-                                                       const cplx<TW>* __restrict__ w,
-                                                       const c128* __restrict__ V, long n, int j,
-                                                       c128* __restrict__ part) {
-  if (ctrl->done) return;
-  __shared__ double sh[16];
-  const long base = (long)blockIdx.x * LZ_ELEMS + threadIdx.x;
-  double wr[LZ_EPT], wi[LZ_EPT];
-#pragma unroll
-  for (int r = 0; r < LZ_EPT; ++r) {
-    const long e = base + (long)r * LZ_BLOCK;
-    wr[r] = e < n ? double(w[e].re) : 0.0;
-    wi[r] = e < n ? double(w[e].im) : 0.0;
-  }
+// This block's partial projections part[blk][i] = sum_e conj(V_i[e]) x[e],
+// i = 0..j, with x held in registers (LZ_EPT elements per thread).  Each
+// projection is reduced inside its wave; one barrier then combines the waves.
+__device__ void block_projections(const c128* __restrict__ V, long n, int j, long base,
+                                  const double* xr, const double* xi,
+                                  c128* __restrict__ part_blk) {
+  constexpr int NW = LZ_BLOCK / 64;
+  __shared__ double sh[NW][LZ_MAXK][2];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int i = 0; i <= j; ++i) {
     const c128* vi = V + (long)i * n;
     double ar = 0, ai = 0;
@@ -269,27 +261,66 @@ __global__ __launch_bounds__(LZ_BLOCK) void lz_project(const LanczosCtrl* ctrl,
       const long e = base + (long)r * LZ_BLOCK;
       if (e < n) {
         const c128 a = vi[e];
-        ar += a.re * wr[r] + a.im * wi[r];
-        ai += a.re * wi[r] - a.im * wr[r];
+        ar += a.re * xr[r] + a.im * xi[r];
+        ai += a.re * xi[r] - a.im * xr[r];
       }
     }
-    ar = block_sum(ar, sh);
-    ai = block_sum(ai, sh);
-    if (threadIdx.x == 0) part[(long)blockIdx.x * LZ_MAXK + i] = c128{ar, ai};
+    ar = wave_sum(ar);
+    ai = wave_sum(ai);
+    if (lane == 0) {
+      sh[wv][i][0] = ar;
+      sh[wv][i][1] = ai;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x <= j) {
+    double ar = 0, ai = 0;
+    for (int w = 0; w < NW; ++w) {
+      ar += sh[w][threadIdx.x][0];
+      ai += sh[w][threadIdx.x][1];
+    }
+    part_blk[threadIdx.x] = c128{ar, ai};
   }
 }
 
-__global__ void lz_reduce_coeffs(const LanczosCtrl* ctrl, const c128* __restrict__ part, int nblk,
-                                 int j, c128* __restrict__ coef) {
+// Partial projections c_i = <V_i, w> over this block's chunk.  Each thread
+// keeps its LZ_EPT elements of w in registers across the i loop.
+template <typename TW>
+__global__ __launch_bounds__(LZ_BLOCK) void lz_project(const LanczosCtrl* ctrl,
+                                                       const cplx<TW>* __restrict__ w,
+                                                       const c128* __restrict__ V, long n, int j,
+                                                       c128* __restrict__ part) {
   if (ctrl->done) return;
-  const int i = threadIdx.x;
-  if (i > j) return;
-  double ar = 0, ai = 0;
-  for (int b = 0; b < nblk; ++b) {
-    ar += part[(long)b * LZ_MAXK + i].re;
-    ai += part[(long)b * LZ_MAXK + i].im;
+  const long base = (long)blockIdx.x * LZ_ELEMS + threadIdx.x;
+  double wr[LZ_EPT], wi[LZ_EPT];
+#pragma unroll
+  for (int r = 0; r < LZ_EPT; ++r) {
+    const long e = base + (long)r * LZ_BLOCK;
+    wr[r] = e < n ? double(w[e].re) : 0.0;
+    wi[r] = e < n ? double(w[e].im) : 0.0;
   }
-  coef[i] = c128{ar, ai};
+  block_projections(V, n, j, base, wr, wi, part + (long)blockIdx.x * LZ_MAXK);
+}
+
+// coef[i] = sum over blocks of part[b][i], i = 0..j: one wave per coefficient,
+// lanes striding over the blocks (fixed order: deterministic).
+constexpr int LZ_RED_THREADS = 1024;
+__global__ __launch_bounds__(LZ_RED_THREADS) void lz_reduce_coeffs(const LanczosCtrl* ctrl,
+                                                                   const c128* __restrict__ part,
+                                                                   int nblk, int j,
+                                                                   c128* __restrict__ coef) {
+  if (ctrl->done) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int i = wv; i <= j; i += LZ_RED_THREADS / 64) {
+    double ar = 0, ai = 0;
+    for (int b = lane; b < nblk; b += 64) {
+      ar += part[(long)b * LZ_MAXK + i].re;
+      ai += part[(long)b * LZ_MAXK + i].im;
+    }
+    ar = wave_sum(ar);
+    ai = wave_sum(ai);
+    if (lane == 0) coef[i] = c128{ar, ai};
+  }
 }
 
 // wout = w - sum_i coef_i V_i (promoting w to FP64), then partial projections
@@ -340,22 +371,7 @@ __global__ __launch_bounds__(LZ_BLOCK) void lz_subtract(const LanczosCtrl* ctrl,
     if (threadIdx.x == 0) part[(long)blockIdx.x * LZ_MAXK] = c128{nrm, 0};
     return;
   }
-  for (int i = 0; i <= j; ++i) {
-    const c128* vi = V + (long)i * n;
-    double ar = 0, ai = 0;
-#pragma unroll
-    for (int r = 0; r < LZ_EPT; ++r) {
-      const long e = base + (long)r * LZ_BLOCK;
-      if (e < n) {
-        const c128 a = vi[e];
-        ar += a.re * xr[r] + a.im * xi[r];
-        ai += a.re * xi[r] - a.im * xr[r];
-      }
-    }
-    ar = block_sum(ar, sh);
-    ai = block_sum(ai, sh);
-    if (threadIdx.x == 0) part[(long)blockIdx.x * LZ_MAXK + i] = c128{ar, ai};
-  }
+  block_projections(V, n, j, base, xr, xi, part + (long)blockIdx.x * LZ_MAXK);
 }
 
 // One wave: close iteration j.  alpha_j = Re(c_j + c'_j), beta_j = |w|;
@@ -471,6 +487,28 @@ __global__ void rho_kernel(const c128* __restrict__ A, const c128* __restrict__ 
       ai = block_sum(ai, sh);
       if (threadIdx.x == 0) rho[p * 2 + q] = c128{ar, ai};
     }
+}
+
+// Observables of the reference's MPS.measure (mps.py:122-140) from per-site
+// 2x2 density matrices: <n_i> = Re rho[1][1] (not normalised), its rounding
+// (half to even, as numpy.round), and -Tr rho log2 rho from the eigenvalues of
+// the Hermitian part, (a+d)/2 +- sqrt(((a-d)/2)^2 + |b|^2), with 0 log 0 = 0.
+// out = [population(n) | d_population(n) | single_site_entropy(n)].
+__global__ void observables_kernel(const c128* __restrict__ rho, int n, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const c128 r00 = rho[4 * i], r01 = rho[4 * i + 1], r10 = rho[4 * i + 2], r11 = rho[4 * i + 3];
+  const double a = r00.re, d = r11.re;
+  const double br = 0.5 * (r01.re + r10.re), bi = 0.5 * (r01.im - r10.im);
+  const double mean = 0.5 * (a + d), half = 0.5 * (a - d);
+  const double rad = sqrt(half * half + br * br + bi * bi);
+  const double lam[2] = {mean + rad, mean - rad};
+  double sse = 0.0;
+  for (int k = 0; k < 2; ++k)
+    if (lam[k] > 0.0) sse -= lam[k] * log2(lam[k]);
+  out[i] = d;
+  out[n + i] = rint(d);
+  out[2 * n + i] = sse;
 }
 
 // ------------------------------------------------------------------ exact 2^N
@@ -637,10 +675,10 @@ void lanczos_iteration_tail(const cplx<TW>* w, long n, int j, double tau, double
   const int nb = lanczos_nblk(n);
   const LanczosCtrl* cc = lw.ctrl;
   hipLaunchKernelGGL(lz_project<TW>, dim3(nb), dim3(LZ_BLOCK), 0, s, cc, w, lw.V, n, j, lw.part);
-  hipLaunchKernelGGL(lz_reduce_coeffs, dim3(1), dim3(LZ_MAXK), 0, s, cc, lw.part, nb, j, lw.coef1);
+  hipLaunchKernelGGL(lz_reduce_coeffs, dim3(1), dim3(LZ_RED_THREADS), 0, s, cc, lw.part, nb, j, lw.coef1);
   hipLaunchKernelGGL(lz_subtract<TW>, dim3(nb), dim3(LZ_BLOCK), 0, s, cc, w, lw.w64, lw.V, n, j,
                      lw.coef1, lw.part, 1);
-  hipLaunchKernelGGL(lz_reduce_coeffs, dim3(1), dim3(LZ_MAXK), 0, s, cc, lw.part, nb, j, lw.coef2);
+  hipLaunchKernelGGL(lz_reduce_coeffs, dim3(1), dim3(LZ_RED_THREADS), 0, s, cc, lw.part, nb, j, lw.coef2);
   hipLaunchKernelGGL(lz_subtract<double>, dim3(nb), dim3(LZ_BLOCK), 0, s, cc, lw.w64, lw.w64, lw.V,
                      n, j, lw.coef2, lw.part, 2);
   hipLaunchKernelGGL(lz_step, dim3(1), dim3(64), 0, s, lw.ctrl, lw.part, nb, lw.coef1, lw.coef2, j,
@@ -657,6 +695,11 @@ template void lanczos_iteration_tail<float>(const c64*, long, int, double, doubl
 void lanczos_finish(long n, c128* y, LanczosWork& lw, hipStream_t s) {
   hipLaunchKernelGGL(lz_combine, dim3(grid_for(n)), dim3(256), 0, s, (const LanczosCtrl*)lw.ctrl,
                      lw.V, n, y);
+  QCA_HIP(hipGetLastError());
+}
+
+void launch_observables(const c128* rho, int n, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(observables_kernel, dim3(ceil_div(n, 64)), dim3(64), 0, s, rho, n, out);
   QCA_HIP(hipGetLastError());
 }
 

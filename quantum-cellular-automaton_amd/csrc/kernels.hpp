@@ -72,6 +72,9 @@ void lanczos_iteration_tail(const cplx<TW>* w, long n, int j, double tau, double
                             LanczosWork& lw, hipStream_t s);
 void lanczos_finish(long n, c128* y, LanczosWork& lw, hipStream_t s);
 
+// population, rounded population and single-site entropy of n sites from their
+// 2x2 density matrices; out holds 3n doubles.
+void launch_observables(const c128* rho, int n, double* out, hipStream_t s);
 void launch_rho(const c128* A, const c128* T, long nxb, c128* rho, hipStream_t s);
 
 int exact_rho_blocks(int ncells);

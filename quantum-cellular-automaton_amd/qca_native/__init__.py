@@ -51,6 +51,7 @@ SIGNATURES = {
     "qca_engine_stats": (c_int, [c_void_p, P(Stats)]),
     "qca_engine_reset_stats": (c_int, [c_void_p]),
     "qca_batch_step": (c_int, [P(c_void_p), c_int, c_int, c_int, P(c_int)]),
+    "qca_engine_observables": (c_int, [c_void_p, P(c_double), P(c_double), P(c_double), P(c_double)]),
     "qca_engine_gpu_ms": (c_int, [c_void_p, P(c_double)]),
     "qca_heff_apply": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_int, c_int, c_int, c_int, c_void_p]),
@@ -161,6 +162,17 @@ class Engine:
         out = np.zeros((self.n, 2, 2), dtype=np.complex128)
         check(lib().qca_engine_measure(self._h, out.ctypes.data))
         return out
+
+    def observables(self, population=None, d_population=None, single_site_entropy=None,
+                    bond_dims=None) -> tuple:
+        """Device-computed rows of the reference's measure(): copies them into
+        the given float64 arrays (row views allowed) and returns the four rows."""
+        bufs = [np.zeros(self.n), np.zeros(self.n), np.zeros(self.n), np.zeros(self.n + 1)]
+        check(lib().qca_engine_observables(self._h, *[b.ctypes.data_as(P(c_double)) for b in bufs]))
+        for dst, b in zip((population, d_population, single_site_entropy, bond_dims), bufs):
+            if dst is not None:
+                dst[...] = b
+        return tuple(bufs)
 
     def bond_dims(self) -> np.ndarray:
         out = (c_int * (self.n + 1))()

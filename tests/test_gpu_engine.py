@@ -287,3 +287,25 @@ def test_a1tdvp_first_steps_match_reference_golden(gpu):
         pops, bds = gpu_trajectory(eng, 2)
         assert np.array_equal(bds, g[f"c{c}_bond_dims"][:2].astype(int))
         assert np.abs(pops - g[f"c{c}_population"][:2]).max() < 1e-9
+
+
+def test_device_observables_match_host_post_processing(gpu):
+    """qca_engine_observables (population, rounded population, single-site
+    entropy, bond dims computed on the device) equals the host post-processing
+    of the same density matrices (write_observables, eigvalsh-based entropy)."""
+    from algorithms.algorithm import write_observables
+    n = 12
+    As = initial("triple", n, 16)
+    eng = make_engine([a.copy() for a in As], AU.mpo_tensors(n, 2, range(2, 5)), 2, 16, 0.1, 1e-10)
+    for step in range(4):
+        rows = [np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n + 1)]
+        eng.observables(*rows)
+        ref = [np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n + 1)]
+        write_observables(eng.rho(), eng.bond_dims(), *ref)
+        assert np.array_equal(rows[0], ref[0])
+        assert np.array_equal(rows[1], ref[1])
+        assert np.abs(rows[2] - ref[2]).max() < 1e-12
+        assert np.array_equal(rows[3], ref[3])
+        if step:
+            assert rows[2].max() > 1e-3        # entanglement has built up
+        eng.step(1)
