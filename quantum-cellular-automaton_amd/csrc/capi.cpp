@@ -531,7 +531,9 @@ int qca_qr_site(int side, const void* A, int chil, int chir, int target, void* A
     const int qcols = std::min(target, rows);
     if (side == 0) qca::launch_to_colmajor((const qca::c128*)A, X, rows, cols, s);
     else qca::launch_site_to_rightqr((const qca::c128*)A, X, chil, chir, s);
-    qca::launch_householder_qr(X, rows, cols, tau, Q, qcols, s);
+    const size_t qws = qca::qr_workspace_elems(rows, cols, std::max(qcols, 1));
+    qca::launch_qr(X, rows, cols, tau, Q, qcols, rows <= 512 ? sbuf<qca::c128>("qr_ws", (long)qws) : nullptr,
+                   qws, s);
     if (side == 0) qca::launch_qr_gather_left(Q, X, (qca::c128*)Anew, (qca::c128*)C, rows, cols, qcols, target, s);
     else qca::launch_qr_gather_right(Q, X, (qca::c128*)Anew, (qca::c128*)C, chir, cols, qcols, target, s);
     QCA_HIP(hipStreamSynchronize(s));

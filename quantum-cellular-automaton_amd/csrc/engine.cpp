@@ -133,6 +133,9 @@ Engine::Engine(int device, const EngineParams& p, const std::vector<const c128*>
   qrQ_ = dmalloc<c128>(4 * cmax * cmax);
   qrTau_ = dmalloc<c128>(2 * cmax);
   cbond_ = dmalloc<c128>(cmax * cmax * 2);
+  qr_ws_elems_ = qr_workspace_elems((int)std::min<long>(2 * cmax, 512), (int)cmax,
+                                    (int)std::min<long>(2 * cmax, 512));
+  qrWs_ = dmalloc<c128>((long)qr_ws_elems_);
   T_ = dmalloc<c128>(tmax);
   U_ = dmalloc<c128>(tmax);
   ws_elems_ = std::max<long>(16 * nmax, 4 * Dmax_ * cmax * cmax);
@@ -427,7 +430,7 @@ void Engine::qr_left_site(int site, int target, c128* C, int& crow, int& ccol, b
   QCA_CHECK(target <= cap_[site + 1], "target bond above capacity");
   launch_to_colmajor(A_[site], svdX_, rows, cols, s_);
   const int qcols = std::min(target, rows);
-  launch_householder_qr(svdX_, rows, cols, qrTau_, qrQ_, qcols, s_);
+  launch_qr(svdX_, rows, cols, qrTau_, qrQ_, qcols, qrWs_, qr_ws_elems_, s_);
   launch_qr_gather_left(qrQ_, svdX_, A_[site], C, rows, cols, qcols, target, s_);
   crow = target;
   ccol = chir;
@@ -441,7 +444,7 @@ void Engine::qr_right_site(int site, int target, c128* C, int& crow, int& ccol, 
   QCA_CHECK(target <= cap_[site], "target bond above capacity");
   launch_site_to_rightqr(A_[site], svdX_, chil, chir, s_);
   const int qcols = std::min(target, rows);
-  launch_householder_qr(svdX_, rows, cols, qrTau_, qrQ_, qcols, s_);
+  launch_qr(svdX_, rows, cols, qrTau_, qrQ_, qcols, qrWs_, qr_ws_elems_, s_);
   launch_qr_gather_right(qrQ_, svdX_, A_[site], C, chir, cols, qcols, target, s_);
   crow = chil;
   ccol = target;

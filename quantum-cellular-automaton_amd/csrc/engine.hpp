@@ -105,6 +105,8 @@ class Engine {
   // workspaces
   c128 *theta_ = nullptr, *theta2_ = nullptr, *svdX_ = nullptr, *qrQ_ = nullptr, *qrTau_ = nullptr;
   c128 *cbond_ = nullptr;
+  c128* qrWs_ = nullptr;        // blocked-QR workspace
+  size_t qr_ws_elems_ = 0;
   void *T_ = nullptr, *U_ = nullptr, *ws_ = nullptr;
   long ws_elems_ = 0;
   ContractWs<double> ws64_;

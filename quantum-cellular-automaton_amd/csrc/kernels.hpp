@@ -124,6 +124,18 @@ void launch_jacobi_svd(c128* X, int rows, int cols, double* sigma, int* perm, Sv
 // -sign(Re alpha)*norm) of the column-major matrix M (rows x cols), in place:
 // R on and above the diagonal, reflectors below it, tau[min(rows, cols)].
 // Q's first qcols columns are written column-major to Q (skipped when null).
+// Blocked Householder QR with LAPACK zgeqrf/zungqr semantics (same reflectors
+// and signs as launch_householder_qr, panels of 16 columns, trailing updates
+// and Q formation as GEMMs): M (rows <= 512 x cols, column-major) -> R on/above
+// the diagonal, reflectors below, tau; Q (rows x qcols) when Q != null.
+// ws holds at least qr_workspace_elems(rows, cols, qcols) complex128.
+size_t qr_workspace_elems(int max_rows, int max_cols, int max_qcols);
+void launch_blocked_qr(c128* M, int rows, int cols, c128* tau, c128* Q, int qcols, c128* ws,
+                       size_t ws_elems, hipStream_t s);
+// QR dispatch: the blocked version for min(rows, cols) > 32 when ws is large
+// enough, else the single-workgroup one.
+void launch_qr(c128* M, int rows, int cols, c128* tau, c128* Q, int qcols, c128* ws,
+               size_t ws_elems, hipStream_t s);
 void launch_householder_qr(c128* M, int rows, int cols, c128* tau, c128* Q, int qcols,
                            hipStream_t s);
 

@@ -565,6 +565,142 @@ __global__ __launch_bounds__(QR_THREADS) void householder_qr_kernel(c128* __rest
   }
 }
 
+// ---------------------------------------------------------- blocked QR
+// One panel of the blocked Householder QR (zgeqrf = zgeqr2 on panels of
+// QRB_PB columns + zlarft + zlarfb): columns [j0, j0+pb) of M (rows x cols,
+// column-major), rows [j0, rows), staged in LDS.  Per column: zlarfg with
+// LAPACK's beta = -sign(Re alpha)*|x|, then H^H = I - conj(tau) v v^H applied to
+// the panel's remaining columns.  Writes R (on/above the diagonal) and v
+// (below) back to M, tau, the unit-diagonal V of the panel into V (same
+// layout as M, zero above the diagonal), and T (pb x pb, column-major with
+// leading dimension QRB_PB, upper triangular) with H_j0 ... H_{j0+pb-1} =
+// I - V T V^H (zlarft, forward, columnwise).
+constexpr int QRB_PB = 16;
+constexpr int QRB_MAXROWS = 512;
+
+__global__ __launch_bounds__(QR_THREADS) void qr_panel_kernel(c128* __restrict__ M, int rows, int j0,
+                                                              int pb, c128* __restrict__ tau,
+                                                              c128* __restrict__ V,
+                                                              c128* __restrict__ T) {
+  __shared__ c128 P[QRB_PB][QRB_MAXROWS];   // panel, local row r = global row j0 + r
+  __shared__ double sh[QR_WAVES];
+  __shared__ c128 s_tau[QRB_PB];
+  __shared__ c128 s_w[QRB_PB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nr = rows - j0;
+  for (int e = tid; e < pb * nr; e += QR_THREADS) {
+    const int c = e / nr, r = e - c * nr;
+    P[c][r] = M[(long)(j0 + c) * rows + j0 + r];
+  }
+  __syncthreads();
+  for (int c = 0; c < pb; ++c) {
+    c128* x = P[c];
+    double part = 0;
+    for (int r = c + 1 + tid; r < nr; r += QR_THREADS) part += x[r].re * x[r].re + x[r].im * x[r].im;
+    const double xnorm2 = qr_block_sum(part, sh);
+    const c128 alpha = x[c];
+    const bool identity = (xnorm2 == 0.0 && alpha.im == 0.0);
+    c128 tj{0.0, 0.0}, scale{0.0, 0.0};
+    double beta = 0.0;
+    if (!identity) {
+      const double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xnorm2);
+      beta = (alpha.re >= 0.0) ? -nrm : nrm;
+      tj = c128{(beta - alpha.re) / beta, -alpha.im / beta};
+      const double dr = alpha.re - beta, di = alpha.im;
+      const double den = dr * dr + di * di;
+      scale = c128{dr / den, -di / den};
+      for (int r = c + 1 + tid; r < nr; r += QR_THREADS) {
+        const c128 v = x[r];
+        x[r] = c128{v.re * scale.re - v.im * scale.im, v.re * scale.im + v.im * scale.re};
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      if (!identity) x[c] = c128{beta, 0.0};
+      s_tau[c] = tj;
+      tau[j0 + c] = tj;
+    }
+    __syncthreads();
+    if (!identity) {
+      const c128 ct{tj.re, -tj.im};
+      for (int cc = c + 1 + wave; cc < pb; cc += QR_WAVES) {
+        c128* y = P[cc];
+        double wr = 0, wi = 0;
+        for (int r = c + lane; r < nr; r += 64) {
+          const c128 v = (r == c) ? c128{1.0, 0.0} : x[r];
+          wr += v.re * y[r].re + v.im * y[r].im;
+          wi += v.re * y[r].im - v.im * y[r].re;
+        }
+        wr = wsum(wr);
+        wi = wsum(wi);
+        const double fr = ct.re * wr - ct.im * wi, fi = ct.re * wi + ct.im * wr;
+        for (int r = c + lane; r < nr; r += 64) {
+          const c128 v = (r == c) ? c128{1.0, 0.0} : x[r];
+          y[r].re -= v.re * fr - v.im * fi;
+          y[r].im -= v.re * fi + v.im * fr;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // write back R / v, and the explicit unit-diagonal V of the panel
+  for (int e = tid; e < pb * rows; e += QR_THREADS) {
+    const int c = e / rows, g = e - c * rows;   // g: global row
+    const int r = g - j0;
+    const long idx = (long)(j0 + c) * rows + g;
+    if (r >= 0) M[idx] = P[c][r];
+    c128 v{0.0, 0.0};
+    if (r == c) v = c128{1.0, 0.0};
+    else if (r > c) v = P[c][r];
+    V[idx] = v;
+  }
+  // T (zlarft forward columnwise): T(i,i) = tau_i,
+  // T(0:i, i) = -tau_i T(0:i, 0:i) (V(:, 0:i)^H v_i)
+  for (int i = 0; i < pb; ++i) {
+    // w_k = V_k^H v_i for k < i (rows >= i of the panel; V_k is zero above k)
+    for (int k = wave; k < i; k += QR_WAVES) {
+      double wr = 0, wi = 0;
+      for (int r = i + lane; r < nr; r += 64) {
+        const c128 a = (r == k) ? c128{1.0, 0.0} : P[k][r];
+        const c128 b = (r == i) ? c128{1.0, 0.0} : P[i][r];
+        wr += a.re * b.re + a.im * b.im;
+        wi += a.re * b.im - a.im * b.re;
+      }
+      wr = wsum(wr);
+      wi = wsum(wi);
+      if (lane == 0) s_w[k] = c128{wr, wi};
+    }
+    __syncthreads();
+    if (tid < pb) {
+      const int k = tid;
+      c128 out{0.0, 0.0};
+      const c128 ti = s_tau[i];
+      if (k < i) {
+        // sum_l T(k, l) w_l for k <= l < i
+        double ar = 0, ai = 0;
+        for (int l = k; l < i; ++l) {
+          const c128 t = T[(long)l * QRB_PB + k], w = s_w[l];
+          ar += t.re * w.re - t.im * w.im;
+          ai += t.re * w.im + t.im * w.re;
+        }
+        out = c128{-(ti.re * ar - ti.im * ai), -(ti.re * ai + ti.im * ar)};
+      } else if (k == i) {
+        out = ti;
+      }
+      T[(long)i * QRB_PB + k] = out;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void set_identity(c128* __restrict__ Q, int rows, int cols) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)rows * cols;
+       e += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(e / rows), r = (int)(e - (long)c * rows);
+    Q[e] = c128{r == c ? 1.0 : 0.0, 0.0};
+  }
+}
+
 }  // namespace
 
 JacobiLayout jacobi_layout(int rows, int cols) {
@@ -638,6 +774,105 @@ void launch_jacobi_svd(c128* X, int rows, int cols, double* sigma, int* perm, Sv
   hipLaunchKernelGGL(jac_finalize, dim3(1), dim3(JAC_NT), 0, s, (const c128*)X, rows, cols, sigma,
                      perm, res, chi_max, eps, rank_cap, (const JacobiCtrl*)ctrl);
   QCA_HIP(hipGetLastError());
+}
+
+size_t qr_workspace_elems(int max_rows, int max_cols, int max_qcols) {
+  const long w = std::max(max_cols, max_qcols);
+  const long panels = ceil_div(std::max(1, std::min(max_rows, max_cols)), QRB_PB);
+  return (size_t)max_rows * max_cols                 // V
+         + panels * QRB_PB * QRB_PB                  // T per panel
+         + 2L * QRB_PB * w                           // W, W2
+         + 32L * QRB_PB * w + 4096;                  // GEMM split-K partials
+}
+
+void launch_blocked_qr(c128* M, int rows, int cols, c128* tau, c128* Q, int qcols, c128* ws,
+                       size_t ws_elems, hipStream_t s) {
+  QCA_CHECK(rows >= 1 && cols >= 1, "empty QR");
+  QCA_CHECK(rows <= QRB_MAXROWS, "blocked QR rows above 512");
+  const int kmax = std::min(rows, cols);
+  const long panels = ceil_div(kmax, QRB_PB);
+  const long wcols = std::max(cols, qcols);
+  c128* V = ws;
+  c128* T = V + (long)rows * cols;
+  c128* W = T + panels * QRB_PB * QRB_PB;
+  c128* W2 = W + QRB_PB * wcols;
+  c128* gws = W2 + QRB_PB * wcols;
+  const long used = gws - ws;
+  QCA_CHECK((long)ws_elems > used + 1024, "QR workspace too small");
+  const long gws_elems = (long)ws_elems - used;
+  // GEMM descriptors on column-major operands: X(r, c) at X + r + c * ld
+  for (int p = 0; p < panels; ++p) {
+    const int j0 = p * QRB_PB, pb = std::min(QRB_PB, kmax - j0);
+    c128* Tp = T + (long)p * QRB_PB * QRB_PB;
+    hipLaunchKernelGGL(qr_panel_kernel, dim3(1), dim3(QR_THREADS), 0, s, M, rows, j0, pb, tau, V, Tp);
+    const int nt = cols - (j0 + pb), nr = rows - j0;
+    if (nt <= 0) continue;
+    const c128* Vp = V + (long)j0 * rows + j0;
+    c128* At = M + (long)(j0 + pb) * rows + j0;
+    // W = V^H A_trail  (pb x nt)
+    GemmDesc g{};
+    g.M = pb; g.N = nt; g.K = nr; g.K2 = nr; g.NB1 = g.NB2 = 1; g.alpha_re = 1.0;
+    g.a_m = rows; g.a_k2 = 1; g.conj_a = 1;
+    g.b_k2 = 1; g.b_n = rows;
+    g.c_m = 1; g.c_n = pb;
+    gemm_c128(g, Vp, At, W, s, gws, gws_elems);
+    // W2 = T^H W
+    GemmDesc h{};
+    h.M = pb; h.N = nt; h.K = pb; h.K2 = pb; h.NB1 = h.NB2 = 1; h.alpha_re = 1.0;
+    h.a_m = QRB_PB; h.a_k2 = 1; h.conj_a = 1;
+    h.b_k2 = 1; h.b_n = pb;
+    h.c_m = 1; h.c_n = pb;
+    gemm_c128(h, Tp, W, W2, s, gws, gws_elems);
+    // A_trail -= V W2
+    GemmDesc u{};
+    u.M = nr; u.N = nt; u.K = pb; u.K2 = pb; u.NB1 = u.NB2 = 1; u.alpha_re = -1.0; u.accumulate = 1;
+    u.a_m = 1; u.a_k2 = rows;
+    u.b_k2 = 1; u.b_n = pb;
+    u.c_m = 1; u.c_n = rows;
+    gemm_c128(u, Vp, W2, At, s, gws, gws_elems);
+  }
+  QCA_HIP(hipGetLastError());
+  if (Q == nullptr || qcols == 0) return;
+  // Q = H_0 ... H_{kk-1} [e_0 .. e_{qcols-1}], block reflectors applied from
+  // the last panel back: Q(j0:, j0:) <- (I - V T V^H) Q(j0:, j0:)  (zungqr)
+  hipLaunchKernelGGL(set_identity, dim3(256), dim3(256), 0, s, Q, rows, qcols);
+  const int kk = std::min(kmax, qcols);
+  for (int p = (int)ceil_div(kk, QRB_PB) - 1; p >= 0; --p) {
+    const int j0 = p * QRB_PB, pb = std::min(QRB_PB, kk - j0);
+    const int nr = rows - j0, nq = qcols - j0;
+    const c128* Vp = V + (long)j0 * rows + j0;
+    const c128* Tp = T + (long)p * QRB_PB * QRB_PB;
+    c128* Qs = Q + (long)j0 * rows + j0;
+    GemmDesc g{};
+    g.M = pb; g.N = nq; g.K = nr; g.K2 = nr; g.NB1 = g.NB2 = 1; g.alpha_re = 1.0;
+    g.a_m = rows; g.a_k2 = 1; g.conj_a = 1;
+    g.b_k2 = 1; g.b_n = rows;
+    g.c_m = 1; g.c_n = pb;
+    gemm_c128(g, Vp, Qs, W, s, gws, gws_elems);
+    GemmDesc h{};
+    h.M = pb; h.N = nq; h.K = pb; h.K2 = pb; h.NB1 = h.NB2 = 1; h.alpha_re = 1.0;
+    h.a_m = 1; h.a_k2 = QRB_PB;
+    h.b_k2 = 1; h.b_n = pb;
+    h.c_m = 1; h.c_n = pb;
+    gemm_c128(h, Tp, W, W2, s, gws, gws_elems);
+    GemmDesc u{};
+    u.M = nr; u.N = nq; u.K = pb; u.K2 = pb; u.NB1 = u.NB2 = 1; u.alpha_re = -1.0; u.accumulate = 1;
+    u.a_m = 1; u.a_k2 = rows;
+    u.b_k2 = 1; u.b_n = pb;
+    u.c_m = 1; u.c_n = rows;
+    gemm_c128(u, Vp, W2, Qs, s, gws, gws_elems);
+  }
+  QCA_HIP(hipGetLastError());
+}
+
+void launch_qr(c128* M, int rows, int cols, c128* tau, c128* Q, int qcols, c128* ws,
+               size_t ws_elems, hipStream_t s) {
+  // one workgroup is quicker while the panels would be few
+  if (ws != nullptr && std::min(rows, cols) > 2 * QRB_PB &&
+      ws_elems >= qr_workspace_elems(rows, cols, qcols))
+    launch_blocked_qr(M, rows, cols, tau, Q, qcols, ws, ws_elems, s);
+  else
+    launch_householder_qr(M, rows, cols, tau, Q, qcols, s);
 }
 
 void launch_householder_qr(c128* M, int rows, int cols, c128* tau, c128* Q, int qcols,
