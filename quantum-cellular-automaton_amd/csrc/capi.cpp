@@ -479,6 +479,8 @@ int qca_svd_split(const void* theta, void* Al, void* Ar, int chil, int chir, int
     static thread_local qca::JacobiWork jw;  // keeps the pinned word pair and the sweep count
     jw.ws_bytes = qca::jacobi_workspace_bytes(rows, cols);
     jw.ws = sbuf<unsigned char>("svd_ws", (long)jw.ws_bytes);
+    jw.pre_elems = qca::jacobi_precond_elems(rows, cols);
+    jw.pre = sbuf<qca::c128>("svd_pre", (long)jw.pre_elems);
     if (!jw.host) QCA_HIP(hipHostMalloc(&jw.host, 64));
     qca::launch_jacobi_svd(X, rows, cols, sigma, perm, res, chi_max, eps, rank_cap, s, jw);
     qca::SvdResult hres;

@@ -159,6 +159,11 @@ Engine::Engine(int device, const EngineParams& p, const std::vector<const c128*>
   svdres_ = dmalloc<SvdResult>(1);
   jw_.ws_bytes = jacobi_workspace_bytes(2 * cmax, 2 * cmax);
   jw_.ws = dmalloc<unsigned char>((long)jw_.ws_bytes);
+  if (2 * cmax > 32) {
+    jw_.pre_elems = jacobi_precond_elems((int)std::min<long>(2 * cmax, 512),
+                                         (int)std::min<long>(2 * cmax, 512));
+    jw_.pre = dmalloc<c128>((long)jw_.pre_elems);
+  }
   QCA_HIP(hipHostMalloc(&jw_.host, 64));
   QCA_HIP(hipHostMalloc((void**)&svdres_host_, sizeof(SvdResult)));
   QCA_HIP(hipHostMalloc((void**)&ctrl_host_, sizeof(LanczosCtrl)));

@@ -103,6 +103,11 @@ struct JacobiWork {
   size_t ws_bytes = 0;
   void* host = nullptr;
   int pred_sweeps = 8;
+  // QR preconditioning (matrices wider than 32 columns): at least
+  // jacobi_precond_elems(rows, cols) complex128, or null to skip it
+  c128* pre = nullptr;
+  size_t pre_elems = 0;
+  int pred_sweeps_pre = 6;
 };
 
 struct JacobiLayout {
@@ -112,6 +117,7 @@ struct JacobiLayout {
 };
 JacobiLayout jacobi_layout(int rows, int cols);
 size_t jacobi_workspace_bytes(int max_rows, int max_cols);
+size_t jacobi_precond_elems(int max_rows, int max_cols);
 
 // One-sided (block) Jacobi SVD of the column-major matrix X (rows x cols,
 // FP64), in place.  On exit the columns are orthogonal; sigma (sorted

@@ -701,6 +701,93 @@ __global__ void set_identity(c128* __restrict__ Q, int rows, int cols) {
   }
 }
 
+// ---------------------------------------------------------- QR preconditioning
+// order[c]: columns of X (rows x cols) by descending norm (ties by index).
+__global__ __launch_bounds__(JAC_NT) void col_norm_order(const c128* __restrict__ X, int rows, int cols,
+                                                         int* __restrict__ order) {
+  __shared__ double skey[JAC_MAXCOLS];
+  __shared__ int sidx[JAC_MAXCOLS];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = JAC_NT / 64;
+  int np2 = 1;
+  while (np2 < cols) np2 <<= 1;
+  for (int c = wave; c < np2; c += nw) {
+    double a = 0;
+    if (c < cols)
+      for (int r = lane; r < rows; r += 64) {
+        const c128 v = X[(long)c * rows + r];
+        a += v.re * v.re + v.im * v.im;
+      }
+    a = wsum(a);
+    if (lane == 0) {
+      skey[c] = (c < cols) ? a : -1.0;
+      sidx[c] = c;
+    }
+  }
+  __syncthreads();
+  for (int size = 2; size <= np2; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int e = tid; e < np2; e += JAC_NT) {
+        const int q = e ^ stride;
+        if (q > e) {
+          const bool desc = ((e & size) == 0);
+          const double ke = skey[e], kq = skey[q];
+          const bool swap = desc ? (ke < kq || (ke == kq && sidx[e] > sidx[q]))
+                                 : (ke > kq || (ke == kq && sidx[e] < sidx[q]));
+          if (swap) {
+            skey[e] = kq; skey[q] = ke;
+            const int ti = sidx[e]; sidx[e] = sidx[q]; sidx[q] = ti;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int e = tid; e < cols; e += JAC_NT) order[e] = sidx[e];
+}
+
+// Xp[:, c] = X[:, order[c]]
+__global__ void permute_cols(const c128* __restrict__ X, int rows, int cols,
+                             const int* __restrict__ order, c128* __restrict__ Xp) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)rows * cols;
+       e += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(e / rows), r = (int)(e - (long)c * rows);
+    Xp[e] = X[(long)order[c] * rows + r];
+  }
+}
+
+// Y = R^H (cols x kq, column-major) from the R on/above the diagonal of the
+// QR-factored M (rows x cols): Y(c, k) = conj(R(k, c)), zero for k > c.
+__global__ void rh_from_qr(const c128* __restrict__ M, int rows, int cols, int kq,
+                           c128* __restrict__ Y) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)cols * kq;
+       e += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(e / cols), c = (int)(e - (long)k * cols);
+    const c128 v = (k <= c) ? M[(long)c * rows + k] : c128{0.0, 0.0};
+    Y[e] = c128{v.re, -v.im};
+  }
+}
+
+// X[:, j] = |Y[:, j]| * Z[:, j], j < kq: the left singular vectors (columns of
+// Z = Q J, orthonormal) scaled by the singular values, the form X J = U S the
+// callers gather from.
+__global__ __launch_bounds__(JAC_NT) void scale_by_norms(const c128* __restrict__ Y, int yrows,
+                                                         const c128* __restrict__ Z, int rows, int kq,
+                                                         c128* __restrict__ X) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int j = blockIdx.x * (JAC_NT / 64) + wave; j < kq; j += gridDim.x * (JAC_NT / 64)) {
+    double a = 0;
+    for (int r = lane; r < yrows; r += 64) {
+      const c128 v = Y[(long)j * yrows + r];
+      a += v.re * v.re + v.im * v.im;
+    }
+    const double sg = sqrt(wsum(a));
+    for (int r = lane; r < rows; r += 64) {
+      const c128 z = Z[(long)j * rows + r];
+      X[(long)j * rows + r] = c128{sg * z.re, sg * z.im};
+    }
+  }
+}
+
 }  // namespace
 
 JacobiLayout jacobi_layout(int rows, int cols) {
@@ -720,18 +807,13 @@ size_t jacobi_workspace_bytes(int max_rows, int max_cols) {
   return jacobi_layout(max_rows, max_cols).bytes + 1024;
 }
 
-void launch_jacobi_svd(c128* X, int rows, int cols, double* sigma, int* perm, SvdResult* res,
-                       int chi_max, double eps, int rank_cap, hipStream_t s, JacobiWork& w) {
-  QCA_CHECK(rows >= 1 && rows <= 512, "SVD rows out of range (1..512)");
-  QCA_CHECK(cols >= 1 && cols <= JAC_MAXCOLS, "SVD cols out of range (1..512)");
-  QCA_CHECK(w.ws != nullptr && w.host != nullptr, "Jacobi workspace not set");
-  const double tol = 1.1102230246251565e-16 * sqrt((double)rows);
-  if (cols <= JAC_LMAX) {
-    hipLaunchKernelGGL(jac_small, dim3(1), dim3(JAC_NT), 0, s, X, rows, cols, sigma, perm, res, chi_max,
-                       eps, rank_cap, tol);
-    QCA_HIP(hipGetLastError());
-    return;
-  }
+namespace {
+
+// Gram-based block Jacobi on the columns of Y (rows x cols), the same block
+// rotations applied to the columns of Z (zrows x cols) when Z != null; leaves
+// the control block in the workspace for the finalize.
+JacobiCtrl* gram_jacobi(c128* Y, int rows, int cols, c128* Z, int zrows, hipStream_t s,
+                        JacobiWork& w) {
   const JacobiLayout l = jacobi_layout(rows, cols);
   QCA_CHECK(l.bytes <= w.ws_bytes, "Jacobi workspace too small");
   static_assert(sizeof(JacobiCtrl) <= 256, "control block exceeds its slot");
@@ -743,21 +825,27 @@ void launch_jacobi_svd(c128* X, int rows, int cols, double* sigma, int* perm, Sv
   QCA_CHECK((reinterpret_cast<unsigned char*>(Jbuf + l.j_elems) - base) <= (long)w.ws_bytes,
             "Jacobi workspace too small");
   const JacobiGeom g{rows, cols, l.bs, l.nb, l.kspl};
+  const JacobiGeom gz{zrows, cols, l.bs, l.nb, ceil_div(zrows, JAC_RC)};
   const int np = l.nb / 2;
-  hipLaunchKernelGGL(jac_init, dim3(1), dim3(JAC_NT), 0, s, X, (long)rows * cols, ctrl);
+  const double tol = 1.1102230246251565e-16 * sqrt((double)rows);
+  hipLaunchKernelGGL(jac_init, dim3(1), dim3(JAC_NT), 0, s, Y, (long)rows * cols, ctrl);
   // launch the sweeps predicted from the previous call, then read the device
   // flag once per chunk (rounds after convergence are no-ops)
   int sweep = 0;
-  int chunk = std::max(2, w.pred_sweeps);
+  int& pred = Z ? w.pred_sweeps_pre : w.pred_sweeps;
+  int chunk = std::max(2, pred);
   while (sweep < JAC_MAX_SWEEPS) {
     for (int c = 0; c < chunk && sweep < JAC_MAX_SWEEPS; ++c, ++sweep) {
       for (int r = 0; r < l.nb - 1; ++r) {
-        hipLaunchKernelGGL(jac_gram, dim3(np, l.kspl), dim3(JAC_NT), 0, s, (const c128*)X, g, r,
+        hipLaunchKernelGGL(jac_gram, dim3(np, l.kspl), dim3(JAC_NT), 0, s, (const c128*)Y, g, r,
                            Gpart, (const JacobiCtrl*)ctrl);
         hipLaunchKernelGGL(jac_inner, dim3(np), dim3(JAC_NT), 0, s, g, sweep, (const c128*)Gpart,
                            Jbuf, rotated, tol, ctrl);
-        hipLaunchKernelGGL(jac_apply, dim3(np, l.kspl), dim3(JAC_NT), 0, s, X, g, r,
+        hipLaunchKernelGGL(jac_apply, dim3(np, l.kspl), dim3(JAC_NT), 0, s, Y, g, r,
                            (const c128*)Jbuf, (const int*)rotated, (const JacobiCtrl*)ctrl);
+        if (Z)
+          hipLaunchKernelGGL(jac_apply, dim3(np, gz.kspl), dim3(JAC_NT), 0, s, Z, gz, r,
+                             (const c128*)Jbuf, (const int*)rotated, (const JacobiCtrl*)ctrl);
       }
       hipLaunchKernelGGL(jac_check, dim3(1), dim3(64), 0, s, ctrl, sweep);
     }
@@ -766,11 +854,63 @@ void launch_jacobi_svd(c128* X, int rows, int cols, double* sigma, int* perm, Sv
     QCA_HIP(hipStreamSynchronize(s));
     const int* hc = reinterpret_cast<const int*>(w.host);
     if (hc[0]) {
-      w.pred_sweeps = hc[1];
+      pred = hc[1];
       break;
     }
     chunk = 2;
   }
+  return ctrl;
+}
+
+}  // namespace
+
+size_t jacobi_precond_elems(int max_rows, int max_cols) {
+  const long kq = std::min(max_rows, max_cols);
+  return (size_t)max_rows * max_cols       // column-permuted copy, then R / reflectors
+         + (size_t)max_rows * kq           // Q, then Q J
+         + (size_t)max_cols * kq           // Y = R^H
+         + (size_t)kq + 64                 // tau
+         + qr_workspace_elems(max_rows, max_cols, (int)kq);
+}
+
+void launch_jacobi_svd(c128* X, int rows, int cols, double* sigma, int* perm, SvdResult* res,
+                       int chi_max, double eps, int rank_cap, hipStream_t s, JacobiWork& w) {
+  QCA_CHECK(rows >= 1 && rows <= 512, "SVD rows out of range (1..512)");
+  QCA_CHECK(cols >= 1 && cols <= JAC_MAXCOLS, "SVD cols out of range (1..512)");
+  QCA_CHECK(w.ws != nullptr && w.host != nullptr, "Jacobi workspace not set");
+  if (cols <= JAC_LMAX) {
+    const double tol = 1.1102230246251565e-16 * sqrt((double)rows);
+    hipLaunchKernelGGL(jac_small, dim3(1), dim3(JAC_NT), 0, s, X, rows, cols, sigma, perm, res, chi_max,
+                       eps, rank_cap, tol);
+    QCA_HIP(hipGetLastError());
+    return;
+  }
+  const int kq = std::min(rows, cols);
+  if (w.pre != nullptr && kq > JAC_LMAX && w.pre_elems >= jacobi_precond_elems(rows, cols)) {
+    // Preconditioned: X P = Q R with the columns by descending norm, one-sided
+    // Jacobi on Y = R^H with the same rotations applied to Q.  Y J = U' S and
+    // Q J = U, the left singular vectors of X, orthonormal by construction.
+    c128* Xp = w.pre;
+    c128* Q = Xp + (long)rows * cols;
+    c128* Y = Q + (long)rows * kq;
+    c128* tau = Y + (long)cols * kq;
+    c128* qws = tau + kq + 64;
+    const size_t qws_elems = w.pre_elems - (size_t)(qws - w.pre);
+    int* order = perm;  // scratch until the finalize writes the permutation
+    hipLaunchKernelGGL(col_norm_order, dim3(1), dim3(JAC_NT), 0, s, (const c128*)X, rows, cols, order);
+    hipLaunchKernelGGL(permute_cols, dim3(256), dim3(256), 0, s, (const c128*)X, rows, cols,
+                       (const int*)order, Xp);
+    launch_qr(Xp, rows, cols, tau, Q, kq, qws, qws_elems, s);
+    hipLaunchKernelGGL(rh_from_qr, dim3(256), dim3(256), 0, s, (const c128*)Xp, rows, cols, kq, Y);
+    JacobiCtrl* ctrl = gram_jacobi(Y, cols, kq, Q, rows, s, w);
+    hipLaunchKernelGGL(jac_finalize, dim3(1), dim3(JAC_NT), 0, s, (const c128*)Y, cols, kq, sigma,
+                       perm, res, chi_max, eps, rank_cap, (const JacobiCtrl*)ctrl);
+    hipLaunchKernelGGL(scale_by_norms, dim3(ceil_div(kq, JAC_NT / 64)), dim3(JAC_NT), 0, s,
+                       (const c128*)Y, cols, (const c128*)Q, rows, kq, X);
+    QCA_HIP(hipGetLastError());
+    return;
+  }
+  JacobiCtrl* ctrl = gram_jacobi(X, rows, cols, nullptr, 0, s, w);
   hipLaunchKernelGGL(jac_finalize, dim3(1), dim3(JAC_NT), 0, s, (const c128*)X, rows, cols, sigma,
                      perm, res, chi_max, eps, rank_cap, (const JacobiCtrl*)ctrl);
   QCA_HIP(hipGetLastError());
