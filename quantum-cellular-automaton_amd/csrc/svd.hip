@@ -583,9 +583,9 @@ __global__ __launch_bounds__(QR_THREADS) void qr_panel_kernel(c128* __restrict__
                                                               c128* __restrict__ V,
                                                               c128* __restrict__ T) {
   __shared__ c128 P[QRB_PB][QRB_MAXROWS];   // panel, local row r = global row j0 + r
-  __shared__ double sh[QR_WAVES];
   __shared__ c128 s_tau[QRB_PB];
-  __shared__ c128 s_w[QRB_PB];
+  __shared__ c128 s_vv[QRB_PB][QRB_PB];     // V^H V of the panel
+  __shared__ double s_xnorm2;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nr = rows - j0;
   for (int e = tid; e < pb * nr; e += QR_THREADS) {
@@ -595,33 +595,39 @@ __global__ __launch_bounds__(QR_THREADS) void qr_panel_kernel(c128* __restrict__
   __syncthreads();
   for (int c = 0; c < pb; ++c) {
     c128* x = P[c];
-    double part = 0;
-    for (int r = c + 1 + tid; r < nr; r += QR_THREADS) part += x[r].re * x[r].re + x[r].im * x[r].im;
-    const double xnorm2 = qr_block_sum(part, sh);
+    // |x[c+1:]|^2 by wave 0 (nr <= 512: at most 8 elements per lane)
+    if (wave == 0) {
+      double part = 0;
+      for (int r = c + 1 + lane; r < nr; r += 64) part += x[r].re * x[r].re + x[r].im * x[r].im;
+      part = wsum(part);
+      if (lane == 0) s_xnorm2 = part;
+    }
+    __syncthreads();
+    // zlarfg, evaluated redundantly by every thread from the shared norm
+    const double xnorm2 = s_xnorm2;
     const c128 alpha = x[c];
     const bool identity = (xnorm2 == 0.0 && alpha.im == 0.0);
-    c128 tj{0.0, 0.0}, scale{0.0, 0.0};
-    double beta = 0.0;
+    c128 tj{0.0, 0.0};
     if (!identity) {
       const double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xnorm2);
-      beta = (alpha.re >= 0.0) ? -nrm : nrm;
+      const double beta = (alpha.re >= 0.0) ? -nrm : nrm;
       tj = c128{(beta - alpha.re) / beta, -alpha.im / beta};
       const double dr = alpha.re - beta, di = alpha.im;
       const double den = dr * dr + di * di;
-      scale = c128{dr / den, -di / den};
+      const c128 scale{dr / den, -di / den};
       for (int r = c + 1 + tid; r < nr; r += QR_THREADS) {
         const c128 v = x[r];
         x[r] = c128{v.re * scale.re - v.im * scale.im, v.re * scale.im + v.im * scale.re};
       }
+      __syncthreads();  // uniform: identity is the same in every thread
+      if (tid == 0) x[c] = c128{beta, 0.0};
     }
-    __syncthreads();
     if (tid == 0) {
-      if (!identity) x[c] = c128{beta, 0.0};
       s_tau[c] = tj;
       tau[j0 + c] = tj;
     }
-    __syncthreads();
     if (!identity) {
+      // C <- (I - conj(tau) v v^H) C on the panel's columns c+1.., v_c = 1
       const c128 ct{tj.re, -tj.im};
       for (int cc = c + 1 + wave; cc < pb; cc += QR_WAVES) {
         c128* y = P[cc];
@@ -654,32 +660,34 @@ __global__ __launch_bounds__(QR_THREADS) void qr_panel_kernel(c128* __restrict__
     else if (r > c) v = P[c][r];
     V[idx] = v;
   }
-  // T (zlarft forward columnwise): T(i,i) = tau_i,
-  // T(0:i, i) = -tau_i T(0:i, 0:i) (V(:, 0:i)^H v_i)
-  for (int i = 0; i < pb; ++i) {
-    // w_k = V_k^H v_i for k < i (rows >= i of the panel; V_k is zero above k)
-    for (int k = wave; k < i; k += QR_WAVES) {
-      double wr = 0, wi = 0;
-      for (int r = i + lane; r < nr; r += 64) {
-        const c128 a = (r == k) ? c128{1.0, 0.0} : P[k][r];
-        const c128 b = (r == i) ? c128{1.0, 0.0} : P[i][r];
-        wr += a.re * b.re + a.im * b.im;
-        wi += a.re * b.im - a.im * b.re;
-      }
-      wr = wsum(wr);
-      wi = wsum(wi);
-      if (lane == 0) s_w[k] = c128{wr, wi};
+  // V^H V of the panel (k < i), one wave per entry
+  for (int e = wave; e < pb * pb; e += QR_WAVES) {
+    const int k = e % pb, i = e / pb;
+    if (k >= i) continue;
+    double wr = 0, wi = 0;
+    for (int r = i + lane; r < nr; r += 64) {   // V_k and V_i overlap on rows >= i
+      const c128 a = (r == k) ? c128{1.0, 0.0} : P[k][r];
+      const c128 b = (r == i) ? c128{1.0, 0.0} : P[i][r];
+      wr += a.re * b.re + a.im * b.im;
+      wi += a.re * b.im - a.im * b.re;
     }
-    __syncthreads();
-    if (tid < pb) {
-      const int k = tid;
+    wr = wsum(wr);
+    wi = wsum(wi);
+    if (lane == 0) s_vv[k][i] = c128{wr, wi};
+  }
+  __syncthreads();
+  // T (zlarft forward columnwise): T(i,i) = tau_i,
+  // T(0:i, i) = -tau_i T(0:i, 0:i) (V(:, 0:i)^H v_i); one lane per row k of T,
+  // the columns in order (T(k, l) for l < i written by the same lane earlier)
+  if (tid < pb) {
+    const int k = tid;
+    for (int i = 0; i < pb; ++i) {
       c128 out{0.0, 0.0};
       const c128 ti = s_tau[i];
       if (k < i) {
-        // sum_l T(k, l) w_l for k <= l < i
         double ar = 0, ai = 0;
         for (int l = k; l < i; ++l) {
-          const c128 t = T[(long)l * QRB_PB + k], w = s_w[l];
+          const c128 t = T[(long)l * QRB_PB + k], w = s_vv[l][i];
           ar += t.re * w.re - t.im * w.im;
           ai += t.re * w.im + t.im * w.re;
         }
@@ -689,7 +697,6 @@ __global__ __launch_bounds__(QR_THREADS) void qr_panel_kernel(c128* __restrict__
       }
       T[(long)i * QRB_PB + k] = out;
     }
-    __syncthreads();
   }
 }
 
