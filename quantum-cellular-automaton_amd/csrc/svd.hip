@@ -254,15 +254,22 @@ __global__ __launch_bounds__(JAC_NT) void jac_inner(JacobiGeom g, int sweep,
   }
 }
 
-// X_p <- X_p J_p on row chunk blockIdx.y of pair blockIdx.x.
-__global__ __launch_bounds__(JAC_NT) void jac_apply(c128* __restrict__ X, JacobiGeom g, int round,
+// X_p <- X_p J_p on row chunk blockIdx.y of pair blockIdx.x; chunks from
+// g.kspl on belong to the second matrix Z (geometry gz), which receives the
+// same rotations (gz.kspl == 0 when there is none).
+__global__ __launch_bounds__(JAC_NT) void jac_apply(c128* __restrict__ Xa, JacobiGeom ga,
+                                                    c128* __restrict__ Za, JacobiGeom gz, int round,
                                                     const c128* __restrict__ Jin,
                                                     const int* __restrict__ rotated,
                                                     const JacobiCtrl* __restrict__ ctrl) {
   if (ctrl->done || rotated[blockIdx.x] == 0) return;
+  const bool second = (int)blockIdx.y >= ga.kspl;
+  c128* __restrict__ X = second ? Za : Xa;
+  const JacobiGeom g = second ? gz : ga;
+  const int kc = second ? (int)blockIdx.y - ga.kspl : (int)blockIdx.y;
   __shared__ c128 tile[JAC_RC * (JAC_LMAX + 1)];
   __shared__ c128 J[JAC_LMAX * JAC_LMAX];
-  const int p = blockIdx.x, kc = blockIdx.y, L = 2 * g.bs;
+  const int p = blockIdx.x, L = 2 * g.bs;
   const int r0 = kc * JAC_RC, nr = min(JAC_RC, g.rows - r0);
   stage_tile(X, g, p, round, r0, nr, tile);
   const c128* jp = Jin + (long)p * L * L;
@@ -832,7 +839,7 @@ JacobiCtrl* gram_jacobi(c128* Y, int rows, int cols, c128* Z, int zrows, hipStre
   QCA_CHECK((reinterpret_cast<unsigned char*>(Jbuf + l.j_elems) - base) <= (long)w.ws_bytes,
             "Jacobi workspace too small");
   const JacobiGeom g{rows, cols, l.bs, l.nb, l.kspl};
-  const JacobiGeom gz{zrows, cols, l.bs, l.nb, ceil_div(zrows, JAC_RC)};
+  const JacobiGeom gz{zrows, cols, l.bs, l.nb, Z ? ceil_div(zrows, JAC_RC) : 0};
   const int np = l.nb / 2;
   const double tol = 1.1102230246251565e-16 * sqrt((double)rows);
   hipLaunchKernelGGL(jac_init, dim3(1), dim3(JAC_NT), 0, s, Y, (long)rows * cols, ctrl);
@@ -848,11 +855,8 @@ JacobiCtrl* gram_jacobi(c128* Y, int rows, int cols, c128* Z, int zrows, hipStre
                            Gpart, (const JacobiCtrl*)ctrl);
         hipLaunchKernelGGL(jac_inner, dim3(np), dim3(JAC_NT), 0, s, g, sweep, (const c128*)Gpart,
                            Jbuf, rotated, tol, ctrl);
-        hipLaunchKernelGGL(jac_apply, dim3(np, l.kspl), dim3(JAC_NT), 0, s, Y, g, r,
+        hipLaunchKernelGGL(jac_apply, dim3(np, l.kspl + gz.kspl), dim3(JAC_NT), 0, s, Y, g, Z, gz, r,
                            (const c128*)Jbuf, (const int*)rotated, (const JacobiCtrl*)ctrl);
-        if (Z)
-          hipLaunchKernelGGL(jac_apply, dim3(np, gz.kspl), dim3(JAC_NT), 0, s, Z, gz, r,
-                             (const c128*)Jbuf, (const int*)rotated, (const JacobiCtrl*)ctrl);
       }
       hipLaunchKernelGGL(jac_check, dim3(1), dim3(64), 0, s, ctrl, sweep);
     }
