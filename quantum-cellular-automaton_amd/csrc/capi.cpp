@@ -328,7 +328,7 @@ int qca_engine_get_site(qca_engine* e, int site, double* out) {
 int qca_engine_stats(qca_engine* e, qca_stats* out) {
   return guarded([&] {
     QCA_CHECK(e && e->eng && out, "null argument");
-    const qca::StepStats& s = e->eng->stats();
+    const qca::StepStats s = e->eng->stats();
     out->matvecs = s.matvecs;
     out->matvec_2site = s.matvec_2site;
     out->matvec_1site = s.matvec_1site;

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 namespace qca {
@@ -178,6 +179,16 @@ Engine::Engine(int device, const EngineParams& p, const std::vector<const c128*>
   lw_.part_norm = dmalloc<double>(lw_.max_blocks);
   lw_.coef1 = dmalloc<c128>(LZ_MAXK);
   lw_.coef2 = dmalloc<c128>(LZ_MAXK);
+  tally_ = dmalloc<KrylovTally>(1);
+  QCA_HIP(hipMemset(tally_, 0, sizeof(KrylovTally)));
+  fused_max_n_ = FUSED_KRYLOV_DEFAULT_N;
+  if (const char* env = std::getenv("QCA_FUSED_KRYLOV_MAX_N")) {
+    char* end = nullptr;
+    const long v = std::strtol(env, &end, 10);
+    QCA_CHECK(end != env && *end == '\0' && v >= 0, "QCA_FUSED_KRYLOV_MAX_N must be a count >= 0");
+    fused_max_n_ = v;
+  }
+  fused_max_n_ = std::min(fused_max_n_, FUSED_KRYLOV_MAX_N);
   measE_ = dmalloc<c128>(cmax * cmax);
   measT_ = dmalloc<c128>(2 * cmax * cmax);
   rho_ = dmalloc<c128>(4L * N_);
@@ -205,6 +216,28 @@ Engine::~Engine() {
 }
 
 void Engine::sync() { QCA_HIP(hipStreamSynchronize(s_)); }
+
+StepStats Engine::stats() {
+  KrylovTally t;
+  QCA_HIP(hipMemcpyAsync(&t, tally_, sizeof(t), hipMemcpyDeviceToHost, s_));
+  sync();
+  StepStats r = stats_;
+  r.matvec_2site += t.matvecs[0];
+  r.matvec_1site += t.matvecs[1];
+  r.matvec_bond += t.matvecs[2];
+  r.matvecs += t.matvecs[0] + t.matvecs[1] + t.matvecs[2];
+  r.lanczos_limit_hits += t.limit_hits;
+  r.flops += t.flops;
+  int khz = 0;
+  QCA_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device_));
+  if (khz > 0) r.krylov_ms += (double)t.ticks / (double)khz;
+  return r;
+}
+
+void Engine::reset_stats() {
+  stats_ = StepStats();
+  QCA_HIP(hipMemsetAsync(tally_, 0, sizeof(KrylovTally), s_));
+}
 
 double Engine::elapsed_ms() {
   float ms = 0.f;
@@ -263,8 +296,48 @@ void Engine::heff(int kind, int site, const cplx<T>* x, cplx<T>* y, int chil, in
                      ws_for<T>(), skip, s_);
 }
 
+// One-launch exponential: no host round trip, totals kept on the device.
+template <typename T>
+void Engine::fused_run(int kind, int site, c128* x, long n, int chil, int chir, double delta) {
+  FusedKrylovArgs<T> a;
+  a.kind = kind;
+  a.chil = chil;
+  a.chir = chir;
+  a.n = n;
+  if (kind == 2) {
+    a.Din = a.Dout = Dl_[site];
+    a.L = Lenv<cplx<T>>(site - 1);
+    a.R = Renv<cplx<T>>(site);
+  } else {
+    const WSparsePair& wp = kind == 0 ? W2f_[site] : Wf_[site];
+    a.Din = Dl_[site];
+    a.Dout = kind == 0 ? Dr_[site + 1] : Dr_[site];
+    a.L = Lenv<cplx<T>>(site - 1);
+    a.R = Renv<cplx<T>>(kind == 0 ? site + 2 : site + 1);
+    a.rowptr = wp.d64.rowptr;
+    a.col = wp.d64.col;
+    a.val = wp.d64.val;
+    a.nnz = wp.d64.nnz;
+  }
+  a.x = x;
+  a.V = lw_.V;
+  a.T1 = reinterpret_cast<c128*>(T_);
+  a.U = reinterpret_cast<c128*>(U_);
+  a.tally = tally_;
+  a.tau = kHalfPi * delta;
+  a.tol = p_.krylov_tol;
+  a.max_m = lw_.max_m;
+  const int D = kind == 2 ? Dl_[site] : std::max(a.Din, a.Dout);
+  a.flops_per_matvec = heff_flops(kind, chil, chir, D);
+  launch_fused_krylov<T>(a, s_);
+}
+
 template <typename T>
 void Engine::lanczos_run(int kind, int site, c128* x, long n, int chil, int chir, double delta) {
+  if (n <= fused_max_n_) {
+    fused_run<T>(kind, site, x, n, chil, chir, delta);
+    return;
+  }
   const double tau = kHalfPi * delta;
   QCA_HIP(hipEventRecord(ev0_, s_));
   lanczos_begin(x, n, lw_, s_);

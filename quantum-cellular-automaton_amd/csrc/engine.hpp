@@ -10,6 +10,9 @@ namespace qca {
 
 enum Precision { PREC_FP64 = 0, PREC_MIXED = 1 };
 
+// Default size limit of the one-launch Krylov exponential (elements).
+constexpr long FUSED_KRYLOV_DEFAULT_N = 1024;
+
 struct EngineParams {
   int ncells = 0;
   int algorithm = 2;       // 1 = 1TDVP, 2 = 2TDVP, 3 = adaptive 1TDVP
@@ -53,8 +56,14 @@ class Engine {
   void get_site(int i, c128* host) const;             // copies (2, chi_i, chi_{i+1})
   void set_stream(hipStream_t s) { s_ = s; }
   hipStream_t stream() const { return s_; }
-  const StepStats& stats() const { return stats_; }
-  void reset_stats() { stats_ = StepStats(); }
+  // Counters since the last reset; waits for the engine's stream so that the
+  // device-side totals of the one-launch exponentials are included.
+  StepStats stats();
+  void reset_stats();
+  // Krylov blocks of at most this many elements use the one-launch
+  // exponential (0: never).  Default FUSED_KRYLOV_DEFAULT_N, or the
+  // QCA_FUSED_KRYLOV_MAX_N environment variable read at creation.
+  long fused_max_n() const { return fused_max_n_; }
 
  private:
   // ---------------- helpers
@@ -124,6 +133,10 @@ class Engine {
   double* obs_ = nullptr;
   int kpred_[3] = {8, 8, 8};
   StepStats stats_;
+  KrylovTally* tally_ = nullptr;               // device totals of the one-launch exponentials
+  long fused_max_n_ = 0;
+  template <typename T> void fused_run(int kind, int site, c128* x, long n, int chil, int chir,
+                                       double delta);
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;  // phase timers
   double elapsed_ms();                         // ev0_ -> ev1_, both complete
 

@@ -72,6 +72,58 @@ void lanczos_iteration_tail(const cplx<TW>* w, long n, int j, double tau, double
                             LanczosWork& lw, hipStream_t s);
 void lanczos_finish(long n, c128* y, LanczosWork& lw, hipStream_t s);
 
+// ---------------------------------------------------------------- one-launch Krylov
+// Longest vector of the one-workgroup exponential (4 residual elements per
+// thread of 1024).
+constexpr long FUSED_KRYLOV_MAX_N = 4096;
+
+// Device-side running totals of the one-launch exponentials of one engine.
+struct KrylovTally {
+  long matvecs[3];          // by kind: 2-site, 1-site, bond
+  long limit_hits;
+  long calls;
+  unsigned long long ticks; // summed kernel time in wall-clock ticks
+  double flops;
+};
+
+// x <- exp(-i tau H_eff) x, H_eff of `kind` (0: 2-site x[q][a][c], q = (p1,p2);
+// 1: 1-site x[p][a][c]; 2: bond x[a][c]) with environments L[a][w][a*] and
+// R[c][v][c*] of element type TE and the sparse MPO (forward form of the
+// merged pair for kind 0, of the site for kind 1; unused for kind 2).  The
+// recurrence runs in FP64 whatever TE is.
+template <typename TE>
+struct FusedKrylovArgs {
+  int kind = 0;
+  int chil = 1, chir = 1;
+  int Din = 1, Dout = 1;
+  long n = 0;
+  const cplx<TE>* L = nullptr;
+  const cplx<TE>* R = nullptr;
+  const int* rowptr = nullptr;
+  const int* col = nullptr;
+  const c128* val = nullptr;
+  int nnz = 0;                 // entries of the sparse MPO
+  c128* x = nullptr;           // in/out, n elements
+  c128* V = nullptr;           // basis, (max_m - 1) * n elements
+  c128* T1 = nullptr;          // dq * Din * chil * chir elements
+  c128* U = nullptr;           // dq * Dout * chil * chir elements
+  LanczosCtrl* ctrl = nullptr; // optional: m, error, limit flag of this call
+  KrylovTally* tally = nullptr;// optional running totals
+  double tau = 0, tol = 0;
+  double flops_per_matvec = 0;
+  int max_m = LZ_MAXK;
+  // Placement in the workgroup's LDS, filled by launch_fused_krylov: byte
+  // offsets into the dynamic LDS (-1: the operand stays in global memory) and
+  // the number of leading basis vectors held there.
+  long offL = -1, offR = -1, offRow = -1, offCol = -1, offVal = -1, offU = -1, offT1 = -1;
+  long offV = -1;
+  long offW = 0;               // residual staging (n elements), always in LDS
+  int vcap = 0;
+};
+
+template <typename TE>
+void launch_fused_krylov(const FusedKrylovArgs<TE>& a, hipStream_t s);
+
 // population, rounded population and single-site entropy of n sites from their
 // 2x2 density matrices; out holds 3n doubles.
 void launch_observables(const c128* rho, int n, double* out, hipStream_t s);

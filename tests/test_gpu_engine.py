@@ -76,8 +76,50 @@ def plist_of(kind, n, seed=0):
     return p
 
 
-@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-n{c[1]}-d{c[2]}-chi{c[7]}" for c in CASES])
+def case_id(c):
+    return f"{c[0]}-n{c[1]}-d{c[2]}-chi{c[7]}"
+
+
+@pytest.mark.parametrize("case", CASES, ids=[case_id(c) for c in CASES])
 def test_engine_matches_oracle(gpu, case):
+    check_against_oracle(case)
+
+
+@pytest.mark.parametrize("limit", [0, 4096], ids=["multi-launch", "one-launch"])
+@pytest.mark.parametrize("case", [CASES[1], CASES[5], CASES[9]], ids=[case_id(c) for c in (CASES[1], CASES[5], CASES[9])])
+def test_both_krylov_paths_match_oracle(gpu, monkeypatch, case, limit):
+    """Every Krylov exponential on the multi-launch path (limit 0), or every
+    one on the one-launch path (limit 4096 covers all blocks at chi <= 32)."""
+    monkeypatch.setenv("QCA_FUSED_KRYLOV_MAX_N", str(limit))
+    check_against_oracle(case)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_one_launch_krylov_equals_multi_launch(gpu, monkeypatch, prec):
+    """The one-launch exponential reproduces the multi-launch one on a chi=32
+    2TDVP run (2-site blocks up to 4096 elements): FP64 to 1e-12 with the same
+    bonds; mixed within the mixed gate of 1e-6.  Its device-side totals
+    (matvecs, time) reach qca_engine_stats."""
+    n, d, iv, chi = 12, 2, range(2, 5), 32
+    runs = {}
+    for limit in (0, 4096):
+        monkeypatch.setenv("QCA_FUSED_KRYLOV_MAX_N", str(limit))
+        e = make_engine(initial("triple", n, chi), AU.mpo_tensors(n, d, iv), 2, chi, 0.1, 1e-10, prec=prec)
+        pops, bds = gpu_trajectory(e, 6)
+        runs[limit] = (pops, bds, e.stats())
+    (p0, b0, s0), (p1, b1, s1) = runs[0], runs[4096]
+    if prec == 0:
+        assert np.array_equal(b0, b1)
+        assert np.abs(p0 - p1).max() < 1e-12
+    else:
+        assert np.abs(p0 - p1).max() < 1e-6
+    assert s1["matvecs"] > 0 and s1["krylov_ms"] > 0
+    assert abs(s1["matvecs"] - s0["matvecs"]) <= 0.05 * s0["matvecs"]
+    assert s1["matvecs"] == s1["matvec_2site"] + s1["matvec_1site"] + s1["matvec_bond"]
+    assert s1["flops"] > 0
+
+
+def check_against_oracle(case):
     alg, n, d, iv, kind, dt, steps, chi, eps = case
     As = initial(kind, n, chi)
     Ws = AU.mpo_tensors(n, d, iv)
