@@ -108,6 +108,13 @@ int qca_engine_reset_stats(qca_engine* e);
 /* Accumulated GPU time (ms) of the engine's stream since the last reset,
  * measured with HIP events around each qca_engine_step call. */
 int qca_engine_gpu_ms(qca_engine* e, double* ms);
+/* Krylov blocks of at most max_n elements (0..4096) use the one-launch
+ * exponential (one workgroup, no host round trip); larger ones use the
+ * multi-launch path.  Default 1024, or QCA_FUSED_KRYLOV_MAX_N at creation.
+ * Raising it trades single-instance latency for throughput when many engines
+ * share the GPU.  No counterpart in the reference (a scheduling choice of this
+ * implementation; results agree to rounding either way). */
+int qca_engine_set_fused_krylov_limit(qca_engine* e, long max_n);
 
 /* Advances each of the n engines by nsteps time steps using nthreads host
  * worker threads (1..256).  Each engine runs on its own stream, so up to

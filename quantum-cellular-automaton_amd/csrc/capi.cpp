@@ -358,6 +358,14 @@ int qca_engine_gpu_ms(qca_engine* e, double* ms) {
   });
 }
 
+int qca_engine_set_fused_krylov_limit(qca_engine* e, long max_n) {
+  return guarded([&] {
+    QCA_CHECK(e, "null engine");
+    QCA_CHECK(max_n >= 0 && max_n <= qca::FUSED_KRYLOV_MAX_N, "limit must be in 0..4096");
+    e->eng->set_fused_max_n(max_n);
+  });
+}
+
 // ------------------------------------------------------------------ primitives
 
 int qca_heff_apply(int precision, int kind, const void* x, void* y, const void* L, const void* R,

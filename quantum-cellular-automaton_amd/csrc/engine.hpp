@@ -64,6 +64,7 @@ class Engine {
   // exponential (0: never).  Default FUSED_KRYLOV_DEFAULT_N, or the
   // QCA_FUSED_KRYLOV_MAX_N environment variable read at creation.
   long fused_max_n() const { return fused_max_n_; }
+  void set_fused_max_n(long n) { fused_max_n_ = n; }
 
  private:
   // ---------------- helpers

@@ -53,6 +53,7 @@ SIGNATURES = {
     "qca_batch_step": (c_int, [P(c_void_p), c_int, c_int, c_int, P(c_int)]),
     "qca_engine_observables": (c_int, [c_void_p, P(c_double), P(c_double), P(c_double), P(c_double)]),
     "qca_engine_gpu_ms": (c_int, [c_void_p, P(c_double)]),
+    "qca_engine_set_fused_krylov_limit": (c_int, [c_void_p, c_long]),
     "qca_heff_apply": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_int, c_int, c_int, c_int, c_void_p]),
     "qca_env_update": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
@@ -122,7 +123,8 @@ class Engine:
     """Owning handle of a device-resident TDVP instance (qca_engine_*)."""
 
     def __init__(self, Ws, As, *, algorithm: int, precision: int, chi_max: int, dt: float,
-                 eps: float, krylov_tol: float = 0.0, device: int = 0):
+                 eps: float, krylov_tol: float = 0.0, device: int = 0,
+                 fused_krylov_max_n: int | None = None):
         n = len(As)
         if len(Ws) != n:
             raise ValueError("one MPO tensor per site is required")
@@ -143,6 +145,13 @@ class Engine:
         h = c_void_p()
         check(lib().qca_engine_create(ctypes.byref(prm), Wp, dl, dr, Ap, ch, ctypes.byref(h)))
         self._h = h
+        if fused_krylov_max_n is not None:
+            self.set_fused_krylov_limit(fused_krylov_max_n)
+
+    def set_fused_krylov_limit(self, max_n: int) -> None:
+        """Krylov blocks of at most ``max_n`` elements use the one-launch
+        exponential (qca_engine_set_fused_krylov_limit)."""
+        check(lib().qca_engine_set_fused_krylov_limit(self._h, int(max_n)))
 
     def close(self) -> None:
         if getattr(self, "_h", None):

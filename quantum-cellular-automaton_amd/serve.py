@@ -179,13 +179,17 @@ def poisson_arrivals(rate: float, window_s: float, seed: int = 5, big_fraction: 
         rid += 1
 
 
-def gpu_engine_factory(precision: int = 1, chi_cap: int | None = None):
-    """Engine factory for the GPU path: one qca_engine per request."""
+def gpu_engine_factory(precision: int = 1, chi_cap: int | None = None,
+                       fused_krylov_max_n: int = 4096):
+    """Engine factory for the GPU path: one qca_engine per request.  Requests
+    share the GPU, so Krylov blocks up to ``fused_krylov_max_n`` elements use
+    the one-launch exponential (throughput over single-request latency)."""
     import qca_native as qn
 
     def make(inst: workloads.Instance):
         chi = inst.chi_max if chi_cap is None else min(inst.chi_max, chi_cap)
         return qn.Engine(workloads.mpo_tensors(inst), workloads.initial_tensors(inst), algorithm=2,
                          precision=precision, chi_max=chi, dt=inst.step_size, eps=inst.svd_epsilon,
-                         krylov_tol=1e-12 if precision == 1 else 1e-14)
+                         krylov_tol=1e-12 if precision == 1 else 1e-14,
+                         fused_krylov_max_n=fused_krylov_max_n)
     return make

@@ -51,3 +51,5 @@ def test_argument_errors_are_reported_without_a_device():
     assert b"null engine in batch" in lib.qca_last_error()
     assert lib.qca_batch_step(None, 0, 5, 1, ctypes.byref(failed)) == 0
     assert failed.value == -1
+    assert lib.qca_engine_set_fused_krylov_limit(None, 1024) == -2
+    assert b"null engine" in lib.qca_last_error()

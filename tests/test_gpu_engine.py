@@ -107,6 +107,14 @@ def test_one_launch_krylov_equals_multi_launch(gpu, monkeypatch, prec):
         e = make_engine(initial("triple", n, chi), AU.mpo_tensors(n, d, iv), 2, chi, 0.1, 1e-10, prec=prec)
         pops, bds = gpu_trajectory(e, 6)
         runs[limit] = (pops, bds, e.stats())
+    # the same through the per-engine setter, starting from the default limit
+    monkeypatch.delenv("QCA_FUSED_KRYLOV_MAX_N")
+    e = make_engine(initial("triple", n, chi), AU.mpo_tensors(n, d, iv), 2, chi, 0.1, 1e-10, prec=prec)
+    e.set_fused_krylov_limit(4096)
+    with pytest.raises(qn.QcaError, match="0..4096"):
+        e.set_fused_krylov_limit(4097)
+    p2, b2 = gpu_trajectory(e, 6)
+    assert np.array_equal(p2, runs[4096][0]) and np.array_equal(b2, runs[4096][1])
     (p0, b0, s0), (p1, b1, s1) = runs[0], runs[4096]
     if prec == 0:
         assert np.array_equal(b0, b1)
