@@ -218,6 +218,9 @@ def main() -> int:
     ap.add_argument("--serve-max-steps", type=int, default=50)
     ap.add_argument("--serve-chi-cap", type=int, default=32)
     ap.add_argument("--serve-big-fraction", type=float, default=0.0)
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batched workload (--batch N instances) and print its object "
+                         "(the command the one-launch Krylov profile is taken with)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the H_eff micro-benchmark and print its roofline object "
                          "(the command the kernel profiles are taken with)")
@@ -226,6 +229,8 @@ def main() -> int:
         print("bench.py measures one GPU: every box in this pool has a single MI355X, and the "
               "2TDVP sweep of one instance does not shard; run with --gpus 1", file=sys.stderr)
         return 2
+    if a.batch_only and not a.batch:
+        a.batch = 32
     if a.batch or a.serve:
         # one hardware queue per worker stream (HIP's default is 4); must be set
         # before the HIP runtime starts
@@ -248,6 +253,9 @@ def main() -> int:
                                        "flops_per_launch": fl,
                                        "mixed_tflops": fl / (ms32 * 1e-3) / 1e12,
                                        "fp64_tflops": fl / (ms64 * 1e-3) / 1e12}}), flush=True)
+        return 0
+    if a.batch_only:
+        print(json.dumps({"batched": batch_bench(torch, a)}), flush=True)
         return 0
     Ws = mpo().W
     psi = initial_state()
