@@ -42,6 +42,10 @@ CASES = [
     ("a1tdvp", 10, 1, range(1, 2), "single", 0.1, 6, 16, 1e-10),
     ("a1tdvp", 11, 2, range(2, 4), "padded", 0.05, 5, 8, 1e-6),
     ("a1tdvp", 12, 2, range(2, 5), "triple", 0.1, 5, 16, 1e-8),
+    # bonds up to 64: splits of up to 128 columns (QR-preconditioned Gram Jacobi),
+    # blocked QR in 1TDVP, both Krylov paths along the chain
+    ("2tdvp", 14, 2, range(2, 5), "triple", 0.1, 10, 64, 1e-10),
+    ("1tdvp", 14, 2, range(2, 4), "padded", 0.05, 6, 64, 0.0),
 ]
 
 ALG_CODE = {"1tdvp": 1, "2tdvp": 2, "a1tdvp": 3}
