@@ -316,13 +316,13 @@ void Engine::fused_run(int kind, int site, c128* x, long n, int chil, int chir, 
     a.R = Renv<cplx<T>>(kind == 0 ? site + 2 : site + 1);
     a.rowptr = wp.d64.rowptr;
     a.col = wp.d64.col;
-    a.val = wp.d64.val;
+    if constexpr (std::is_same<T, double>::value) a.val = wp.d64.val; else a.val = wp.d32.val;
     a.nnz = wp.d64.nnz;
   }
   a.x = x;
   a.V = lw_.V;
-  a.T1 = reinterpret_cast<c128*>(T_);
-  a.U = reinterpret_cast<c128*>(U_);
+  a.T1 = reinterpret_cast<cplx<T>*>(T_);
+  a.U = reinterpret_cast<cplx<T>*>(U_);
   a.tally = tally_;
   a.tau = kHalfPi * delta;
   a.tol = p_.krylov_tol;

@@ -90,7 +90,7 @@ struct KrylovTally {
 // 1: 1-site x[p][a][c]; 2: bond x[a][c]) with environments L[a][w][a*] and
 // R[c][v][c*] of element type TE and the sparse MPO (forward form of the
 // merged pair for kind 0, of the site for kind 1; unused for kind 2).  The
-// recurrence runs in FP64 whatever TE is.
+// contractions run in TE, the recurrence in FP64 whatever TE is.
 template <typename TE>
 struct FusedKrylovArgs {
   int kind = 0;
@@ -101,12 +101,12 @@ struct FusedKrylovArgs {
   const cplx<TE>* R = nullptr;
   const int* rowptr = nullptr;
   const int* col = nullptr;
-  const c128* val = nullptr;
+  const cplx<TE>* val = nullptr;
   int nnz = 0;                 // entries of the sparse MPO
   c128* x = nullptr;           // in/out, n elements
   c128* V = nullptr;           // basis, (max_m - 1) * n elements
-  c128* T1 = nullptr;          // dq * Din * chil * chir elements
-  c128* U = nullptr;           // dq * Dout * chil * chir elements
+  cplx<TE>* T1 = nullptr;      // dq * Din * chil * chir elements
+  cplx<TE>* U = nullptr;       // dq * Dout * chil * chir elements
   LanczosCtrl* ctrl = nullptr; // optional: m, error, limit flag of this call
   KrylovTally* tally = nullptr;// optional running totals
   double tau = 0, tol = 0;

@@ -5,8 +5,10 @@
 // with the Lanczos scheme of lautils.py:44-81, entirely inside one workgroup:
 //
 //   * H_eff: L-contraction -> sparse MPO mixing -> R-contraction, the three
-//     stages separated by workgroup barriers; the result lands in the
-//     registers of the threads that own its elements;
+//     stages separated by workgroup barriers, in the environments' type
+//     (complex128, or complex64 on FP32 for the mixed path, as on the
+//     multi-launch path); the result lands, in FP64, in the registers of the
+//     threads that own its elements;
 //   * recurrence: FP64, classical Gram-Schmidt applied twice against the whole
 //     basis (the multi-launch path's scheme), the residual in registers
 //     (element e belongs to thread e % 1024);
@@ -37,10 +39,11 @@ __device__ __forceinline__ double fk_wave_sum(double v) {
   return v;
 }
 
-template <typename TE>
-__device__ __forceinline__ c128 ld(const cplx<TE>* __restrict__ p, unsigned i) {
-  const cplx<TE> v = p[i];
-  return c128{double(v.re), double(v.im)};
+// Element i of p in the compute type TC.
+template <typename TC, typename TS>
+__device__ __forceinline__ cplx<TC> ldc(const cplx<TS>* __restrict__ p, unsigned i) {
+  const cplx<TS> v = p[i];
+  return cplx<TC>{TC(v.re), TC(v.im)};
 }
 
 struct FkShared {
@@ -59,9 +62,9 @@ struct FkCtx {
   const cplx<TE>* R;
   const int* rowptr;
   const int* col;
-  const c128* val;
-  c128* T1;
-  c128* U;
+  const cplx<TE>* val;
+  cplx<TE>* T1;
+  cplx<TE>* U;
   c128* Vl;  // leading vcap basis vectors (LDS)
   c128* Vg;  // the others (global)
   c128* wbuf;  // residual staging for the projections (LDS, n elements)
@@ -147,7 +150,8 @@ __device__ void fk_subtract(const FkCtx<TE>& f, int j, double* wr, double* wi, c
 }
 
 // Sum over the S lanes of a group (S a power of two); every lane gets it.
-__device__ __forceinline__ void fk_segment_sum(double& re, double& im, unsigned S) {
+template <typename T>
+__device__ __forceinline__ void fk_segment_sum(T& re, T& im, unsigned S) {
   for (int d = (int)S >> 1; d > 0; d >>= 1) {
     re += __shfl_xor(re, d, 64);
     im += __shfl_xor(im, d, 64);
@@ -181,8 +185,8 @@ struct FkGemm {
   unsigned cm, cb;
 };
 
-template <int TM, int TN, typename TA, typename TB>
-__device__ void fk_gemm_tiled(const FkGemm& g, const cplx<TA>* A, const cplx<TB>* B, c128* C,
+template <int TM, int TN, typename TC, typename TA, typename TB, typename TO>
+__device__ void fk_gemm_tiled(const FkGemm& g, const cplx<TA>* A, const cplx<TB>* B, cplx<TO>* C,
                               unsigned S) {
   const unsigned tm = (g.M + TM - 1) / TM, tn = (g.N + TN - 1) / TN;
   const unsigned per_b = tm * tn, tiles = per_b * g.NB;
@@ -197,11 +201,11 @@ __device__ void fk_gemm_tiled(const FkGemm& g, const cplx<TA>* A, const cplx<TB>
     for (int i = 0; i < TM; ++i) aoff[i] = b * g.ab + min(m0 + i, g.M - 1) * g.am;
 #pragma unroll
     for (int j = 0; j < TN; ++j) boff[j] = b * g.bb + min(n0 + j, g.N - 1) * g.bn;
-    double cr[TM][TN], ci[TM][TN];
+    TC cr[TM][TN], ci[TM][TN];
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
-      for (int j = 0; j < TN; ++j) cr[i][j] = ci[i][j] = 0.0;
+      for (int j = 0; j < TN; ++j) cr[i][j] = ci[i][j] = TC(0);
     unsigned k1 = 0, k2 = s;
     while (k2 >= g.K2 && k1 < g.K1) {
       k2 -= g.K2;
@@ -209,11 +213,11 @@ __device__ void fk_gemm_tiled(const FkGemm& g, const cplx<TA>* A, const cplx<TB>
     }
     for (unsigned k = s; k < K; k += S) {
       const unsigned ka = k1 * g.ak1 + k2 * g.ak2, kb = k1 * g.bk1 + k2 * g.bk2;
-      c128 av[TM], bv[TN];
+      cplx<TC> av[TM], bv[TN];
 #pragma unroll
-      for (int i = 0; i < TM; ++i) av[i] = ld(A, aoff[i] + ka);
+      for (int i = 0; i < TM; ++i) av[i] = ldc<TC>(A, aoff[i] + ka);
 #pragma unroll
-      for (int j = 0; j < TN; ++j) bv[j] = ld(B, boff[j] + kb);
+      for (int j = 0; j < TN; ++j) bv[j] = ldc<TC>(B, boff[j] + kb);
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -237,7 +241,7 @@ __device__ void fk_gemm_tiled(const FkGemm& g, const cplx<TA>* A, const cplx<TB>
 #pragma unroll
         for (int j = 0; j < TN; ++j)
           if (m0 + i < g.M && n0 + j < g.N)
-            C[b * g.cb + (m0 + i) * g.cm + n0 + j] = c128{cr[i][j], ci[i][j]};
+            C[b * g.cb + (m0 + i) * g.cm + n0 + j] = cplx<TO>{TO(cr[i][j]), TO(ci[i][j])};
     }
   }
 }
@@ -245,8 +249,8 @@ __device__ void fk_gemm_tiled(const FkGemm& g, const cplx<TA>* A, const cplx<TB>
 // Picks the tile (4x2, 2x2 or 1x1: the largest that still gives most of the
 // workgroup a tile; 4x4 would not fit the 128 registers of a 1024-thread
 // workgroup) and the K split, then runs the GEMM.
-template <typename TA, typename TB>
-__device__ void fk_gemm(const FkGemm& g, const cplx<TA>* A, const cplx<TB>* B, c128* C) {
+template <typename TC, typename TA, typename TB, typename TO>
+__device__ void fk_gemm(const FkGemm& g, const cplx<TA>* A, const cplx<TB>* B, cplx<TO>* C) {
   auto tiles = [&](unsigned tm, unsigned tn) {
     return ((g.M + tm - 1) / tm) * ((g.N + tn - 1) / tn) * g.NB;
   };
@@ -255,9 +259,9 @@ __device__ void fk_gemm(const FkGemm& g, const cplx<TA>* A, const cplx<TB>* B, c
     while (S < 16 && n * S * 2 <= FK_THREADS) S *= 2;
     return S;
   };
-  if (tiles(4, 2) >= FK_THREADS / 2) fk_gemm_tiled<4, 2>(g, A, B, C, 1);
-  else if (tiles(2, 2) >= FK_THREADS / 4) fk_gemm_tiled<2, 2>(g, A, B, C, split(tiles(2, 2)));
-  else fk_gemm_tiled<1, 1>(g, A, B, C, split(tiles(1, 1)));
+  if (tiles(4, 2) >= FK_THREADS / 2) fk_gemm_tiled<4, 2, TC>(g, A, B, C, 1);
+  else if (tiles(2, 2) >= FK_THREADS / 4) fk_gemm_tiled<2, 2, TC>(g, A, B, C, split(tiles(2, 2)));
+  else fk_gemm_tiled<1, 1, TC>(g, A, B, C, split(tiles(1, 1)));
 }
 
 template <typename TE>
@@ -274,7 +278,7 @@ __device__ void fk_heff(const FusedKrylovArgs<TE>& a, const FkCtx<TE>& f, const 
     g.am = 1; g.ak1 = 0; g.ak2 = Din * chil; g.ab = 0;
     g.bk1 = 0; g.bk2 = chir; g.bn = 1; g.bb = lr;
     g.cm = chir; g.cb = Din * lr;
-    fk_gemm(g, f.L, v, f.T1);
+    fk_gemm<TE>(g, f.L, v, f.T1);
   }
   __syncthreads();
   if (a.kind == 2) {
@@ -284,7 +288,7 @@ __device__ void fk_heff(const FusedKrylovArgs<TE>& a, const FkCtx<TE>& f, const 
     g.am = chir; g.ak1 = lr; g.ak2 = 1; g.ab = 0;
     g.bk1 = chir; g.bk2 = Din * chir; g.bn = 1; g.bb = 0;
     g.cm = chir; g.cb = 0;
-    fk_gemm(g, f.T1, f.R, f.wbuf);
+    fk_gemm<TE>(g, f.T1, f.R, f.wbuf);
     fk_collect(f, yr, yi);
     return;
   }
@@ -292,16 +296,16 @@ __device__ void fk_heff(const FusedKrylovArgs<TE>& a, const FkCtx<TE>& f, const 
   const unsigned e2 = dq * Dout * lr;
   for (unsigned e = threadIdx.x; e < e2; e += FK_THREADS) {
     const unsigned o = e / lr, xc = e - o * lr;
-    double re = 0, im = 0;
+    TE re = 0, im = 0;
     const int z1 = f.rowptr[o + 1];
     for (int zz = f.rowptr[o]; zz < z1; ++zz) {
-      const c128 wv = f.val[zz];
-      const c128 t = f.T1[(unsigned)f.col[zz] * lr + xc];
+      const cplx<TE> wv = f.val[zz];
+      const cplx<TE> t = f.T1[(unsigned)f.col[zz] * lr + xc];
       re += wv.re * t.re - wv.im * t.im;
       im += wv.re * t.im + wv.im * t.re;
     }
     const unsigned qp = o / Dout, vv = o - qp * Dout;
-    f.U[(qp * lr + xc) * Dout + vv] = c128{re, im};
+    f.U[(qp * lr + xc) * Dout + vv] = cplx<TE>{re, im};
   }
   __syncthreads();
   // y[(q',x)][yy] = sum_{(c,v)} U[(q',x)][(c,v)] R[(c,v)][yy]
@@ -311,7 +315,7 @@ __device__ void fk_heff(const FusedKrylovArgs<TE>& a, const FkCtx<TE>& f, const 
     g.am = chir * Dout; g.ak1 = 0; g.ak2 = 1; g.ab = 0;
     g.bk1 = 0; g.bk2 = chir; g.bn = 1; g.bb = 0;
     g.cm = chir; g.cb = 0;
-    fk_gemm(g, f.U, f.R, f.wbuf);
+    fk_gemm<TE>(g, f.U, f.R, f.wbuf);
   }
   fk_collect(f, yr, yi);
 }
@@ -382,7 +386,7 @@ __global__ __launch_bounds__(FK_THREADS) void fused_krylov_kernel(FusedKrylovArg
   const int dq = a.kind == 0 ? 4 : (a.kind == 1 ? 2 : 1);
   if (a.offL >= 0) stage(arena + a.offL, a.L, (long)a.chil * a.Din * a.chil * sizeof(cplx<TE>));
   if (a.offR >= 0) stage(arena + a.offR, a.R, (long)a.chir * a.Dout * a.chir * sizeof(cplx<TE>));
-  if (a.offVal >= 0) stage(arena + a.offVal, a.val, (long)a.nnz * sizeof(c128));
+  if (a.offVal >= 0) stage(arena + a.offVal, a.val, (long)a.nnz * sizeof(cplx<TE>));
   if (a.offRow >= 0) {
     int* rp = reinterpret_cast<int*>(arena + a.offRow);
     for (int i = threadIdx.x; i <= dq * a.Dout; i += FK_THREADS) rp[i] = a.rowptr[i];
@@ -527,10 +531,10 @@ void launch_fused_krylov(const FusedKrylovArgs<TE>& a, hipStream_t s) {
   if (a.kind != 2) {
     b.offRow = take((long)(dq * a.Dout + 1) * sizeof(int));
     b.offCol = take((long)a.nnz * sizeof(int));
-    b.offVal = take((long)a.nnz * sizeof(c128));
-    b.offU = take((long)dq * a.Dout * lr * sizeof(c128));
+    b.offVal = take((long)a.nnz * sizeof(cplx<TE>));
+    b.offU = take((long)dq * a.Dout * lr * sizeof(cplx<TE>));
   }
-  b.offT1 = take((long)dq * a.Din * lr * sizeof(c128));
+  b.offT1 = take((long)dq * a.Din * lr * sizeof(cplx<TE>));
   const long vbytes = a.n * (long)sizeof(c128);
   b.vcap = (int)std::min<long>(a.max_m - 1, (FK_LDS_BYTES - off) / vbytes);
   b.offV = b.vcap > 0 ? off : -1;
