@@ -77,7 +77,8 @@ typedef struct qca_stats {
   long site_updates, svd_sweeps, lanczos_limit_hits, svd_calls;
   double flops;      /* algorithmic contraction flops executed */
   double svd_ms;     /* device time of the 2-site SVD splits */
-  double krylov_ms;  /* device time of the Krylov exponentials */
+  double krylov_ms;  /* device time of the Krylov exponentials (event-timed on the
+                        multi-launch path, kernel clock on the one-launch path) */
 } qca_stats;
 
 /* W[i]: host, complex128 (2,2,Wdl[i],Wdr[i]); A[i]: host, complex128
@@ -103,6 +104,10 @@ int qca_engine_observables(qca_engine* e, double* population, double* d_populati
 int qca_engine_bond_dims(qca_engine* e, int* out);
 /* Copy site tensor i to host complex128 (2, chi_i, chi_{i+1}). */
 int qca_engine_get_site(qca_engine* e, int site, double* out);
+/* Counters since the last reset.  Waits for the engine's stream: the
+ * one-launch Krylov exponentials keep their counts (and kernel time, in
+ * krylov_ms) on the device.  No counterpart in the reference (instrumentation
+ * for the site-update and flop accounting of the benchmark). */
 int qca_engine_stats(qca_engine* e, qca_stats* out);
 int qca_engine_reset_stats(qca_engine* e);
 /* Accumulated GPU time (ms) of the engine's stream since the last reset,
