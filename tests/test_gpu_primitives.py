@@ -154,6 +154,10 @@ def test_krylov_expm(gpu, prec, kind, chil, chir, delta):
 @pytest.mark.parametrize("right", [1, 0])
 @pytest.mark.parametrize("chil,chir,chi_max,eps", [(1, 1, 4, 1e-10), (2, 4, 8, 1e-10), (8, 8, 16, 1e-10),
                                                    (8, 8, 5, 1e-10), (16, 32, 64, 1e-3),
+                                                   # one workgroup with four 16-column blocks,
+                                                   # full and ragged (34..48 columns)
+                                                   (32, 32, 64, 1e-10), (24, 20, 64, 1e-10),
+                                                   (17, 20, 64, 1e-10),
                                                    (64, 64, 128, 1e-10), (128, 128, 128, 1e-10)])
 def test_svd_split(gpu, right, chil, chir, chi_max, eps):
     import torch
