@@ -218,6 +218,9 @@ __global__ void splitk_reduce(GemmDesc d, const cplx<T>* __restrict__ ws, cplx<T
     const long r = idx - zb * mn;
     const int m = (int)(r / d.N), n = (int)(r - (long)m * d.N);
     T xr = 0, xi = 0;
+    // unrolled so that several partial loads are in flight; the sum stays in
+    // split order
+#pragma unroll 8
     for (int s = 0; s < splits; ++s) {
       const cplx<T> v = ws[((long)(zb * splits + s) * d.M + m) * d.N + n];
       xr += v.re;
@@ -248,7 +251,7 @@ void gemm_launch(const GemmDesc& d, const cplx<T>* A, const cplx<T>* B, cplx<T>*
   // split K when the output tiles alone cannot occupy the chip
   int splits = 1;
   if (tiles < 192 && d.K >= 4 * BK) {
-    splits = (int)std::min<long>((384 + tiles - 1) / tiles, d.K / (2 * BK));
+    splits = (int)std::min<long>((256 + tiles - 1) / tiles, d.K / (2 * BK));
     while (splits > 1 && (long)splits * batch * d.M * d.N > ws_elems) --splits;
     if (splits < 1) splits = 1;
   }
