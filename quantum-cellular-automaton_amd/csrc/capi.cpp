@@ -119,6 +119,7 @@ qca::LanczosWork lanczos_ws(long n, qca::LanczosCtrl** ctrl_host, int max_m = qc
   lw.V = sbuf<qca::c128>("lz_V", (long)max_m * n);
   lw.w64 = sbuf<qca::c128>("lz_w", n);
   lw.part = sbuf<qca::c128>("lz_part", (long)lw.max_blocks * qca::LZ_MAXK);
+  lw.part2 = sbuf<qca::c128>("lz_part2", (long)lw.max_blocks * qca::LZ_MAXK);
   lw.part_norm = sbuf<double>("lz_pnorm", lw.max_blocks);
   lw.coef1 = sbuf<qca::c128>("lz_c1", qca::LZ_MAXK);
   lw.coef2 = sbuf<qca::c128>("lz_c2", qca::LZ_MAXK);

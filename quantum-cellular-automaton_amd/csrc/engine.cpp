@@ -176,6 +176,7 @@ Engine::Engine(int device, const EngineParams& p, const std::vector<const c128*>
   lw_.V = dmalloc<c128>((long)LZ_MAXK * nmax);
   lw_.w64 = dmalloc<c128>(nmax);
   lw_.part = dmalloc<c128>((long)lw_.max_blocks * LZ_MAXK);
+  lw_.part2 = dmalloc<c128>((long)lw_.max_blocks * LZ_MAXK);
   lw_.part_norm = dmalloc<double>(lw_.max_blocks);
   lw_.coef1 = dmalloc<c128>(LZ_MAXK);
   lw_.coef2 = dmalloc<c128>(LZ_MAXK);

@@ -304,40 +304,39 @@ __global__ __launch_bounds__(LZ_BLOCK) void lz_project(const LanczosCtrl* ctrl,
   block_projections(V, n, j, base, wr, wi, part + (long)blockIdx.x * LZ_MAXK);
 }
 
-// coef[i] = sum over blocks of part[b][i], i = 0..j: one wave per coefficient,
-// lanes striding over the blocks (fixed order: deterministic).
-constexpr int LZ_RED_THREADS = 1024;
-__global__ __launch_bounds__(LZ_RED_THREADS) void lz_reduce_coeffs(const LanczosCtrl* ctrl,
-                                                                   const c128* __restrict__ part,
-                                                                   int nblk, int j,
-                                                                   c128* __restrict__ coef) {
-  if (ctrl->done) return;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int i = wv; i <= j; i += LZ_RED_THREADS / 64) {
-    double ar = 0, ai = 0;
-    for (int b = lane; b < nblk; b += 64) {
-      ar += part[(long)b * LZ_MAXK + i].re;
-      ai += part[(long)b * LZ_MAXK + i].im;
-    }
-    ar = wave_sum(ar);
-    ai = wave_sum(ai);
-    if (lane == 0) coef[i] = c128{ar, ai};
-  }
-}
-
-// wout = w - sum_i coef_i V_i (promoting w to FP64), then partial projections
-// of wout (pass 1) or its partial squared norm (pass 2).
+// coef_i = sum over blocks of part_in[b][i] (i = 0..j), reduced by every block
+// in one fixed order (one wave per coefficient, lanes striding over the
+// blocks, then a butterfly: deterministic), so no separate reduction launch is
+// needed; block 0 also stores them in coef_out.  Then wout = w - sum_i coef_i V_i (promoting w to
+// FP64), and the partial projections of wout (pass 1) or its partial squared
+// norm (pass 2) go to part_out, a different buffer from part_in.
 template <typename TW>
 __global__ __launch_bounds__(LZ_BLOCK) void lz_subtract(const LanczosCtrl* ctrl,
                                                         const cplx<TW>* __restrict__ w,
                                                         c128* __restrict__ wout,
                                                         const c128* __restrict__ V, long n, int j,
-                                                        const c128* __restrict__ coef,
-                                                        c128* __restrict__ part, int pass) {
+                                                        const c128* __restrict__ part_in, int nblk,
+                                                        c128* __restrict__ coef_out,
+                                                        c128* __restrict__ part_out, int pass) {
   if (ctrl->done) return;
   __shared__ double sh[16];
   __shared__ c128 cs[LZ_MAXK];
-  if (threadIdx.x <= j) cs[threadIdx.x] = coef[threadIdx.x];
+  {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int i = wv; i <= j; i += LZ_BLOCK / 64) {
+      double ar = 0, ai = 0;
+      for (int b = lane; b < nblk; b += 64) {
+        ar += part_in[(long)b * LZ_MAXK + i].re;
+        ai += part_in[(long)b * LZ_MAXK + i].im;
+      }
+      ar = wave_sum(ar);
+      ai = wave_sum(ai);
+      if (lane == 0) {
+        cs[i] = c128{ar, ai};
+        if (blockIdx.x == 0) coef_out[i] = c128{ar, ai};
+      }
+    }
+  }
   __syncthreads();
   const long base = (long)blockIdx.x * LZ_ELEMS + threadIdx.x;
   double xr[LZ_EPT], xi[LZ_EPT];
@@ -370,10 +369,10 @@ __global__ __launch_bounds__(LZ_BLOCK) void lz_subtract(const LanczosCtrl* ctrl,
   }
   if (pass == 2) {
     nrm = block_sum(nrm, sh);
-    if (threadIdx.x == 0) part[(long)blockIdx.x * LZ_MAXK] = c128{nrm, 0};
+    if (threadIdx.x == 0) part_out[(long)blockIdx.x * LZ_MAXK] = c128{nrm, 0};
     return;
   }
-  block_projections(V, n, j, base, xr, xi, part + (long)blockIdx.x * LZ_MAXK);
+  block_projections(V, n, j, base, xr, xi, part_out + (long)blockIdx.x * LZ_MAXK);
 }
 
 // One wave: close iteration j.  alpha_j = Re(c_j + c'_j), beta_j = |w|;
@@ -420,9 +419,8 @@ __global__ void lz_step(LanczosCtrl* ctrl, const c128* __restrict__ part_norm, i
       ti = ni;
       ar += tr;
       ai += ti;
-      double mag = fabs(tr) + fabs(ti);
-      for (int o = 32; o > 0; o >>= 1) mag = fmax(mag, __shfl_xor(mag, o, 64));
-      if (mag < 1e-18) break;
+      // stop once every row's term is below 1e-18 (a NaN keeps going, bounded)
+      if (__ballot(!(fabs(tr) + fabs(ti) < 1e-18)) == 0) break;
     }
     zr = ar;
     zi = ai;
@@ -677,12 +675,10 @@ void lanczos_iteration_tail(const cplx<TW>* w, long n, int j, double tau, double
   const int nb = lanczos_nblk(n);
   const LanczosCtrl* cc = lw.ctrl;
   hipLaunchKernelGGL(lz_project<TW>, dim3(nb), dim3(LZ_BLOCK), 0, s, cc, w, lw.V, n, j, lw.part);
-  hipLaunchKernelGGL(lz_reduce_coeffs, dim3(1), dim3(LZ_RED_THREADS), 0, s, cc, lw.part, nb, j, lw.coef1);
   hipLaunchKernelGGL(lz_subtract<TW>, dim3(nb), dim3(LZ_BLOCK), 0, s, cc, w, lw.w64, lw.V, n, j,
-                     lw.coef1, lw.part, 1);
-  hipLaunchKernelGGL(lz_reduce_coeffs, dim3(1), dim3(LZ_RED_THREADS), 0, s, cc, lw.part, nb, j, lw.coef2);
+                     (const c128*)lw.part, nb, lw.coef1, lw.part2, 1);
   hipLaunchKernelGGL(lz_subtract<double>, dim3(nb), dim3(LZ_BLOCK), 0, s, cc, lw.w64, lw.w64, lw.V,
-                     n, j, lw.coef2, lw.part, 2);
+                     n, j, (const c128*)lw.part2, nb, lw.coef2, lw.part, 2);
   hipLaunchKernelGGL(lz_step, dim3(1), dim3(64), 0, s, lw.ctrl, lw.part, nb, lw.coef1, lw.coef2, j,
                      tau, tol, lw.max_m);
   hipLaunchKernelGGL(lz_scale_next, dim3(grid_for(n)), dim3(256), 0, s, cc, lw.w64,

@@ -28,6 +28,7 @@ struct LanczosWork {
   c128* V = nullptr;      // [LZ_MAXK][max_n] Krylov basis
   c128* w64 = nullptr;    // [max_n] orthogonalised residual
   c128* part = nullptr;   // [max_blocks][LZ_MAXK] partial projections
+  c128* part2 = nullptr;  // same size: the second pass's partials (read and written apart)
   double* part_norm = nullptr;
   c128* coef1 = nullptr;  // [LZ_MAXK]
   c128* coef2 = nullptr;
