@@ -341,19 +341,20 @@ void Engine::lanczos_run(int kind, int site, c128* x, long n, int chil, int chir
   }
   const double tau = kHalfPi * delta;
   QCA_HIP(hipEventRecord(ev0_, s_));
-  lanczos_begin(x, n, lw_, s_);
+  constexpr bool mixed = !std::is_same<T, double>::value;
+  // the mixed path's matvec operand (complex64 copy of V_j) comes with each new
+  // basis vector from the Lanczos kernels
+  lanczos_begin(x, n, lw_, s_, mixed ? x32_ : nullptr);
   int j = 0;
   int chunk = std::max(2, kpred_[kind]);
   while (true) {
     for (int c = 0; c < chunk && j < lw_.max_m - 2; ++c, ++j) {
-      const c128* vj = lw_.V + (long)j * n;
       if constexpr (std::is_same<T, double>::value) {
-        heff<double>(kind, site, vj, y64_, chil, chir);
+        heff<double>(kind, site, lw_.V + (long)j * n, y64_, chil, chir);
         lanczos_iteration_tail<double>(y64_, n, j, tau, p_.krylov_tol, lw_, s_);
       } else {
-        launch_convert_c128_c64(vj, x32_, n, s_, lanczos_done_flag(lw_));
         heff<float>(kind, site, x32_, y32_, chil, chir);
-        lanczos_iteration_tail<float>(y32_, n, j, tau, p_.krylov_tol, lw_, s_);
+        lanczos_iteration_tail<float>(y32_, n, j, tau, p_.krylov_tol, lw_, s_, x32_);
       }
     }
     QCA_HIP(hipMemcpyAsync(ctrl_host_, lw_.ctrl, sizeof(LanczosCtrl), hipMemcpyDeviceToHost, s_));

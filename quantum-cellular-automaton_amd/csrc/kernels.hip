@@ -239,11 +239,14 @@ __global__ void lz_init_finalize(LanczosCtrl* ctrl, const double* __restrict__ p
 }
 
 __global__ void lz_scale_first(const c128* __restrict__ x, c128* __restrict__ v0, long n,
-                               const LanczosCtrl* ctrl) {
+                               const LanczosCtrl* ctrl, c64* __restrict__ v32) {
   const double b = ctrl->beta0;
   const double inv = b > 0 ? 1.0 / b : 0.0;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    v0[i] = c128{x[i].re * inv, x[i].im * inv};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const c128 v{x[i].re * inv, x[i].im * inv};
+    v0[i] = v;
+    if (v32) v32[i] = c64{float(v.re), float(v.im)};
+  }
 }
 
 // This block's partial projections part[blk][i] = sum_e conj(V_i[e]) x[e],
@@ -375,18 +378,20 @@ __global__ __launch_bounds__(LZ_BLOCK) void lz_subtract(const LanczosCtrl* ctrl,
   block_projections(V, n, j, base, xr, xi, part_out + (long)blockIdx.x * LZ_MAXK);
 }
 
+// beta_j = |w| from the per-block partial norms (one wave, fixed order).
+__device__ __forceinline__ double lz_beta(const c128* __restrict__ part_norm, int nblk) {
+  double s = 0;
+  for (int b = threadIdx.x & 63; b < nblk; b += 64) s += part_norm[(long)b * LZ_MAXK].re;
+  return sqrt(wave_sum(s));
+}
+
 // One wave: close iteration j.  alpha_j = Re(c_j + c'_j), beta_j = |w|;
 // z = exp(-i tau T_{j+1}) e_0 by scaled Taylor steps on the tridiagonal
 // (lane = row); converged when beta0*beta_j*|z_j| < tol*beta0.
-__global__ void lz_step(LanczosCtrl* ctrl, const c128* __restrict__ part_norm, int nblk,
-                        const c128* __restrict__ coef1, const c128* __restrict__ coef2, int j,
-                        double tau, double tol, int max_m) {
-  if (ctrl->done) return;
-  const int l = threadIdx.x;  // 64 lanes
-  double s = 0;
-  for (int b = l; b < nblk; b += 64) s += part_norm[(long)b * LZ_MAXK].re;
-  s = wave_sum(s);
-  const double beta = sqrt(s);
+__device__ void lz_close_iteration(LanczosCtrl* ctrl, double beta, const c128* __restrict__ coef1,
+                                   const c128* __restrict__ coef2, int j, double tau, double tol,
+                                   int max_m) {
+  const int l = threadIdx.x & 63;
   const double alpha_j = coef1[j].re + coef2[j].re;
   if (l == 0) {
     ctrl->alpha[j] = alpha_j;
@@ -442,12 +447,34 @@ __global__ void lz_step(LanczosCtrl* ctrl, const c128* __restrict__ part_norm, i
   }
 }
 
-__global__ void lz_scale_next(const LanczosCtrl* ctrl, const c128* __restrict__ w,
-                              c128* __restrict__ vnext, long n, int j) {
+// Closes iteration j and forms the next basis vector in one launch: every
+// block reduces beta_j itself (same order everywhere), the first wave of block
+// 0 closes the iteration, and all blocks write V_{j+1} = w / beta_j, plus its
+// complex64 copy for the mixed path's next matvec when v32 is given.  Blocks
+// that start after convergence skip the write; V_{j+1} is not read then.
+__global__ __launch_bounds__(256) void lz_step_scale(LanczosCtrl* ctrl,
+                                                     const c128* __restrict__ part_norm, int nblk,
+                                                     const c128* __restrict__ coef1,
+                                                     const c128* __restrict__ coef2, int j,
+                                                     double tau, double tol, int max_m,
+                                                     const c128* __restrict__ w,
+                                                     c128* __restrict__ vnext,
+                                                     c64* __restrict__ v32, long n) {
   if (ctrl->done) return;
-  const double inv = 1.0 / ctrl->beta[j];
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    vnext[i] = c128{w[i].re * inv, w[i].im * inv};
+  __shared__ double sbeta;
+  if (threadIdx.x < 64) {
+    const double b = lz_beta(part_norm, nblk);
+    if (threadIdx.x == 0) sbeta = b;
+  }
+  __syncthreads();
+  const double beta = sbeta;
+  if (blockIdx.x == 0 && threadIdx.x < 64) lz_close_iteration(ctrl, beta, coef1, coef2, j, tau, tol, max_m);
+  const double inv = 1.0 / beta;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const c128 v{w[i].re * inv, w[i].im * inv};
+    vnext[i] = v;
+    if (v32) v32[i] = c64{float(v.re), float(v.im)};
+  }
 }
 
 // y = beta0 * sum_{i<m} z_i V_i   (a zero input stays zero)
@@ -658,19 +685,19 @@ int lanczos_nblk(long n) { return (int)std::max<long>(1, (n + LZ_ELEMS - 1) / LZ
 
 const int* lanczos_done_flag(const LanczosWork& lw) { return &lw.ctrl->done; }
 
-void lanczos_begin(const c128* x, long n, LanczosWork& lw, hipStream_t s) {
+void lanczos_begin(const c128* x, long n, LanczosWork& lw, hipStream_t s, c64* v32) {
   const int nb = lanczos_nblk(n);
   QCA_CHECK(nb <= lw.max_blocks, "Lanczos vector longer than the workspace");
   hipLaunchKernelGGL(lz_norm_partials, dim3(nb), dim3(LZ_BLOCK), 0, s, x, n, lw.part_norm);
   hipLaunchKernelGGL(lz_init_finalize, dim3(1), dim3(64), 0, s, lw.ctrl, lw.part_norm, nb);
   hipLaunchKernelGGL(lz_scale_first, dim3(grid_for(n)), dim3(256), 0, s, x, lw.V, n,
-                     (const LanczosCtrl*)lw.ctrl);
+                     (const LanczosCtrl*)lw.ctrl, v32);
   QCA_HIP(hipGetLastError());
 }
 
 template <typename TW>
 void lanczos_iteration_tail(const cplx<TW>* w, long n, int j, double tau, double tol,
-                            LanczosWork& lw, hipStream_t s) {
+                            LanczosWork& lw, hipStream_t s, c64* v32) {
   QCA_CHECK(j + 1 < lw.max_m && lw.max_m <= LZ_MAXK, "Krylov dimension limit");
   const int nb = lanczos_nblk(n);
   const LanczosCtrl* cc = lw.ctrl;
@@ -679,16 +706,15 @@ void lanczos_iteration_tail(const cplx<TW>* w, long n, int j, double tau, double
                      (const c128*)lw.part, nb, lw.coef1, lw.part2, 1);
   hipLaunchKernelGGL(lz_subtract<double>, dim3(nb), dim3(LZ_BLOCK), 0, s, cc, lw.w64, lw.w64, lw.V,
                      n, j, (const c128*)lw.part2, nb, lw.coef2, lw.part, 2);
-  hipLaunchKernelGGL(lz_step, dim3(1), dim3(64), 0, s, lw.ctrl, lw.part, nb, lw.coef1, lw.coef2, j,
-                     tau, tol, lw.max_m);
-  hipLaunchKernelGGL(lz_scale_next, dim3(grid_for(n)), dim3(256), 0, s, cc, lw.w64,
-                     lw.V + (long)(j + 1) * n, n, j);
+  hipLaunchKernelGGL(lz_step_scale, dim3(grid_for(n)), dim3(256), 0, s, lw.ctrl,
+                     (const c128*)lw.part, nb, (const c128*)lw.coef1, (const c128*)lw.coef2, j, tau,
+                     tol, lw.max_m, (const c128*)lw.w64, lw.V + (long)(j + 1) * n, v32, n);
   QCA_HIP(hipGetLastError());
 }
 template void lanczos_iteration_tail<double>(const c128*, long, int, double, double, LanczosWork&,
-                                             hipStream_t);
+                                             hipStream_t, c64*);
 template void lanczos_iteration_tail<float>(const c64*, long, int, double, double, LanczosWork&,
-                                            hipStream_t);
+                                            hipStream_t, c64*);
 
 void lanczos_finish(long n, c128* y, LanczosWork& lw, hipStream_t s) {
   hipLaunchKernelGGL(lz_combine, dim3(grid_for(n)), dim3(256), 0, s, (const LanczosCtrl*)lw.ctrl,

@@ -67,10 +67,12 @@ void launch_gather_isometry(const c128* X, const int* perm, const double* sigma,
 
 int lanczos_nblk(long n);
 const int* lanczos_done_flag(const LanczosWork& lw);
-void lanczos_begin(const c128* x, long n, LanczosWork& lw, hipStream_t s);
+// v32 (optional): also write each new basis vector as complex64 there, the
+// operand of the mixed path's next matvec.
+void lanczos_begin(const c128* x, long n, LanczosWork& lw, hipStream_t s, c64* v32 = nullptr);
 template <typename TW>
 void lanczos_iteration_tail(const cplx<TW>* w, long n, int j, double tau, double tol,
-                            LanczosWork& lw, hipStream_t s);
+                            LanczosWork& lw, hipStream_t s, c64* v32 = nullptr);
 void lanczos_finish(long n, c128* y, LanczosWork& lw, hipStream_t s);
 
 // ---------------------------------------------------------------- one-launch Krylov
