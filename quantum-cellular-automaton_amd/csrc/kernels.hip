@@ -213,7 +213,7 @@ __global__ void qr_gather_right(const c128* __restrict__ Q, const c128* __restri
 
 // ------------------------------------------------------------------ lanczos
 constexpr int LZ_BLOCK = 256;
-constexpr int LZ_EPT = 2;                     // vector elements per thread
+constexpr int LZ_EPT = 1;                     // vector elements per thread
 constexpr int LZ_ELEMS = LZ_BLOCK * LZ_EPT;   // elements per block in the partial reductions
 
 __global__ void lz_norm_partials(const c128* __restrict__ x, long n, double* __restrict__ part) {
