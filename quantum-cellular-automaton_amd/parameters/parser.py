@@ -61,7 +61,7 @@ def build_argparser() -> argparse.ArgumentParser:
                         "matrix cores with FP64 Krylov/SVD (mixed)")
     g.add_argument("--device", dest="DEVICE", type=int, default=0, help="HIP device ordinal")
     g.add_argument("--krylov-tol", dest="KRYLOV_TOL", type=float, default=0.0,
-                   help="Krylov exponential tolerance (0 = 1e-14 for fp64, 1e-12 for mixed)")
+                   help="Krylov exponential tolerance (0 = 1e-14 for fp64, 1e-10 for mixed)")
     g = p.add_argument_group("Plot")
     g.add_argument("--plotting-frequency", dest="PLOTTING_FREQUENCY", type=float, default=1.0,
                    help="Frequency at which time steps are plotted")

@@ -190,6 +190,6 @@ def gpu_engine_factory(precision: int = 1, chi_cap: int | None = None,
         chi = inst.chi_max if chi_cap is None else min(inst.chi_max, chi_cap)
         return qn.Engine(workloads.mpo_tensors(inst), workloads.initial_tensors(inst), algorithm=2,
                          precision=precision, chi_max=chi, dt=inst.step_size, eps=inst.svd_epsilon,
-                         krylov_tol=1e-12 if precision == 1 else 1e-14,
+                         krylov_tol=1e-10 if precision == 1 else 1e-14,
                          fused_krylov_max_n=fused_krylov_max_n)
     return make
