@@ -150,6 +150,41 @@ def batch_bench(torch, a):
             "speedup_over_sequential": rates["batched"] / rates["sequential"]}
 
 
+def tdvp1_bench(torch, a):
+    """1TDVP at fixed chi = 256: 101 cells, distance 2, interval [2,3], each
+    cell alive with p = 0.3 (seed 3), bonds padded to min(2^i, 2^(N-i), 256)
+    with complex Gaussian filler x 1e-8 (seed 3), dt 0.05.  Both precisions
+    run `a.tdvp1` time steps from the same state; 2N site-updates per step."""
+    import qca_native as qn
+    import states
+    from parameters import Rules
+    from tensor_networks import MPO
+    n, chi, dt = 101, 256, 0.05
+    Ws = MPO.hamiltonian_from_rules(Rules(n, range(2, 4), 2, False)).W
+    psi = states.padded(states.rand_plist(n, 0.3, seed=3), chi, 1e-8, seed=3)
+    out, pops = {}, {}
+    for name, prec, tol in (("fp64", 0, 1e-14), ("mixed", 1, a.mixed_krylov_tol)):
+        log(f"1TDVP chi={chi}: building the {name} engine")
+        e = qn.Engine(Ws, psi.A, algorithm=1, precision=prec, chi_max=chi, dt=dt, eps=1e-10,
+                      krylov_tol=tol)
+        e.reset_stats()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e.step(a.tdvp1)
+        torch.cuda.synchronize()
+        dt_s = time.perf_counter() - t0
+        st = e.stats()
+        pops[name] = e.rho()[:, 1, 1].real.copy()
+        out[name] = {"s_per_step": dt_s / a.tdvp1, "site_updates_per_s": st["site_updates"] / dt_s,
+                     "krylov_ms_per_step": st["krylov_ms"] / a.tdvp1, "matvecs_per_step": st["matvecs"] / a.tdvp1}
+        log(f"1TDVP chi={chi} {name}: {dt_s / a.tdvp1:.3f} s/step")
+        e.close()
+    return {"workload": "1TDVP, 101 cells, distance 2, interval [2,3], p=0.3 product state (seed 3) "
+                        "padded to chi 256 with 1e-8 filler (seed 3), dt 0.05",
+            "steps": a.tdvp1, **out,
+            "max_abs_dn_mixed_vs_fp64": float(np.abs(pops["mixed"] - pops["fp64"]).max())}
+
+
 def serve_bench(a):
     """A served window: Poisson arrivals at `a.serve` requests/s for
     `a.serve_window` s (default_rng(5)), each request a batched-workload draw
@@ -221,6 +256,8 @@ def main() -> int:
     ap.add_argument("--serve-max-steps", type=int, default=50)
     ap.add_argument("--serve-chi-cap", type=int, default=32)
     ap.add_argument("--serve-big-fraction", type=float, default=0.0)
+    ap.add_argument("--tdvp1", type=int, default=0,
+                    help="also time N steps of the 1TDVP chi=256 workload (101 cells), both precisions")
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the batched workload (--batch N instances) and print its object "
                          "(the command the one-launch Krylov profile is taken with)")
@@ -302,6 +339,7 @@ def main() -> int:
     ms64, _ = heff2_roofline(torch, 0)
     ach32 = fl / (ms32 * 1e-3) / 1e12
     ach64 = fl / (ms64 * 1e-3) / 1e12
+    tdvp1 = tdvp1_bench(torch, a) if a.tdvp1 else None
     batched = batch_bench(torch, a) if a.batch else None
     served = serve_bench(a) if a.serve else None
     cpu = None
@@ -344,6 +382,7 @@ def main() -> int:
                      "fp64": {"achieved": ach64, "peak": 78.6, "frac": ach64 / 78.6, "ms_per_launch": ms64,
                               "traffic": HEFF2_TRAFFIC_BYTES["fp64"]}},
         "cpu_baseline": cpu,
+        "tdvp1_chi256": tdvp1,
         "batched": batched,
         "served": served,
     }
