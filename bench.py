@@ -185,6 +185,36 @@ def tdvp1_bench(torch, a):
             "max_abs_dn_mixed_vs_fp64": float(np.abs(pops["mixed"] - pops["fp64"]).max())}
 
 
+def exact_apply_bench(torch, ncells: int, reps: int = 20):
+    """The exact path's closed-form H application (reference exact.py via
+    mpo.py:202-228) on a 2^N state, distance 2, interval [2,4]: HIP-event time
+    per application on its stream, and the HBM rate against its algorithmic
+    bytes (psi read once, H psi written once)."""
+    import qca_native as qn
+    out = {"ncells": ncells, "rule": "distance 2, interval [2,4]"}
+    for name, prec, ct in (("fp64", 0, torch.complex128), ("mixed", 1, torch.complex64)):
+        g = torch.Generator(device="cuda").manual_seed(0)
+        psi = torch.randn(1 << ncells, dtype=ct, device="cuda", generator=g)
+        y = torch.empty_like(psi)
+        st = torch.cuda.current_stream()
+        for _ in range(2):
+            qn.call("qca_exact_apply_h", prec, qn.ptr(psi), qn.ptr(y), ncells, 2, 2, 5, st.cuda_stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            qn.call("qca_exact_apply_h", prec, qn.ptr(psi), qn.ptr(y), ncells, 2, 2, 5, st.cuda_stream)
+        e1.record(st)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        nbytes = 2 * psi.element_size() * (1 << ncells)
+        out[name] = {"ms_per_apply": ms, "algorithmic_bytes": nbytes,
+                     "achieved_GBps": nbytes / (ms * 1e-3) / 1e9,
+                     "frac_of_8TBps": nbytes / (ms * 1e-3) / 8e12}
+        log(f"exact H apply 2^{ncells} {name}: {ms:.3f} ms, {nbytes / (ms * 1e-3) / 1e9:.0f} GB/s")
+        del psi, y
+    return out
+
+
 def serve_bench(a):
     """A served window: Poisson arrivals at `a.serve` requests/s for
     `a.serve_window` s (default_rng(5)), each request a batched-workload draw
@@ -258,6 +288,8 @@ def main() -> int:
     ap.add_argument("--serve-big-fraction", type=float, default=0.0)
     ap.add_argument("--tdvp1", type=int, default=0,
                     help="also time N steps of the 1TDVP chi=256 workload (101 cells), both precisions")
+    ap.add_argument("--exact-apply", type=int, default=0,
+                    help="also time the exact path's 2^N H application at N cells")
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the batched workload (--batch N instances) and print its object "
                          "(the command the one-launch Krylov profile is taken with)")
@@ -340,6 +372,7 @@ def main() -> int:
     ach32 = fl / (ms32 * 1e-3) / 1e12
     ach64 = fl / (ms64 * 1e-3) / 1e12
     tdvp1 = tdvp1_bench(torch, a) if a.tdvp1 else None
+    exact = exact_apply_bench(torch, a.exact_apply) if a.exact_apply else None
     batched = batch_bench(torch, a) if a.batch else None
     served = serve_bench(a) if a.serve else None
     cpu = None
@@ -383,6 +416,7 @@ def main() -> int:
                               "traffic": HEFF2_TRAFFIC_BYTES["fp64"]}},
         "cpu_baseline": cpu,
         "tdvp1_chi256": tdvp1,
+        "exact_apply": exact,
         "batched": batched,
         "served": served,
     }
