@@ -1,7 +1,7 @@
 """MPS container at the host boundary (reference tensor_networks/mps.py).
 
 Holds the site tensors A[p][a][b] (2, chi_l, chi_r) as complex128 numpy arrays
-and the file/vector conversions the drivers use.  Time evolution and
+and the file/vector conversions the simulation loop uses.  Time evolution and
 measurement of an MPS run on the GPU through ``algorithms.TDVP``; this module
 only moves data in and out.
 """
