@@ -540,11 +540,25 @@ __global__ void observables_kernel(const c128* __restrict__ rho, int n, double* 
 
 // ------------------------------------------------------------------ exact 2^N
 // (H psi)[x] = sum_i [popcount(x & nbr_i) in [lo, hi)] psi[x ^ bit_i]
+// A workgroup owns one tile of 2^EX_TILE_BITS consecutive basis states (the
+// whole vector when it is shorter), staged in LDS: partners across the low
+// bits come from the tile, only flips of the higher bits gather from memory.
+constexpr int EX_TILE_BITS = 10;
+constexpr int EX_THREADS = 256;
+
 template <typename T>
-__global__ void exact_apply_kernel(const cplx<T>* __restrict__ psi, cplx<T>* __restrict__ out,
-                                   int ncells, int distance, int lo, int hi) {
-  const long dim = 1L << ncells;
-  for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < dim; x += (long)gridDim.x * blockDim.x) {
+__global__ __launch_bounds__(EX_THREADS) void exact_apply_kernel(const cplx<T>* __restrict__ psi,
+                                                                 cplx<T>* __restrict__ out,
+                                                                 int ncells, int distance, int lo,
+                                                                 int hi) {
+  __shared__ cplx<T> tile[1 << EX_TILE_BITS];
+  const int tbits = min(ncells, EX_TILE_BITS);
+  const long tsize = 1L << tbits;
+  const long base = (long)blockIdx.x << tbits;
+  for (long t = threadIdx.x; t < tsize; t += EX_THREADS) tile[t] = psi[base + t];
+  __syncthreads();
+  for (long t = threadIdx.x; t < tsize; t += EX_THREADS) {
+    const long x = base + t;
     T ar = 0, ai = 0;
     for (int i = 0; i < ncells; ++i) {
       const int jlo = max(0, i - distance), jhi = min(ncells - 1, i + distance);
@@ -555,7 +569,7 @@ __global__ void exact_apply_kernel(const cplx<T>* __restrict__ psi, cplx<T>* __r
       const unsigned long long self = 1ULL << (ncells - 1 - i);
       const int cnt = __popcll((unsigned long long)x & span & ~self);
       if (cnt >= lo && cnt < hi) {
-        const cplx<T> v = psi[x ^ (long)self];
+        const cplx<T> v = ((long)self < tsize) ? tile[t ^ (long)self] : psi[x ^ (long)self];
         ar += v.re;
         ai += v.im;
       }
@@ -735,9 +749,11 @@ void launch_rho(const c128* A, const c128* T, long nxb, c128* rho, hipStream_t s
 template <typename T>
 void launch_exact_apply(const cplx<T>* psi, cplx<T>* out, int ncells, int distance, int lo, int hi,
                         hipStream_t s) {
-  const long dim = 1L << ncells;
-  hipLaunchKernelGGL(exact_apply_kernel<T>, dim3(grid_for(dim, 256, 65536)), dim3(256), 0, s, psi,
-                     out, ncells, distance, lo, hi);
+  QCA_CHECK(ncells >= 1 && ncells <= 34, "ncells out of range for the state vector");
+  const long tiles = 1L << std::max(0, ncells - EX_TILE_BITS);
+  QCA_CHECK(tiles <= 0x7fffffffL, "too many tiles for one launch");
+  hipLaunchKernelGGL(exact_apply_kernel<T>, dim3((unsigned)tiles), dim3(EX_THREADS), 0, s, psi, out,
+                     ncells, distance, lo, hi);
   QCA_HIP(hipGetLastError());
 }
 template void launch_exact_apply<double>(const c128*, c128*, int, int, int, int, hipStream_t);
