@@ -43,18 +43,30 @@ struct Mfma<float> {
   static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-template <typename T, bool A_MCONT, bool B_NCONT>
-__global__ __launch_bounds__(256) void gemm_kernel(GemmDesc d, const cplx<T>* __restrict__ A,
-                                                   const cplx<T>* __restrict__ B,
-                                                   cplx<T>* __restrict__ C,
-                                                   cplx<T>* __restrict__ ws, int splits,
-                                                   int k_per_split) {
+// KG K-groups of 256 threads share a workgroup: group g takes the K-tiles
+// g, g + KG, ... of the workgroup's range into its own LDS buffers, and the
+// groups' partial tiles are summed through LDS at the end.  With KG = 2 a CU
+// holding one workgroup runs two waves per SIMD, so one group's staging and
+// barrier waits overlap the other's MFMAs.
+template <typename T, bool A_MCONT, bool B_NCONT, int KG>
+__global__ __launch_bounds__(256 * KG) void gemm_kernel(GemmDesc d, const cplx<T>* __restrict__ A,
+                                                        const cplx<T>* __restrict__ B,
+                                                        cplx<T>* __restrict__ C,
+                                                        cplx<T>* __restrict__ ws, int splits,
+                                                        int k_per_split) {
   using acc_t = typename Mfma<T>::acc_t;
+  static_assert(BM == BN, "one LDS stride for both operands");
   if (d.skip && *d.skip) return;
-  __shared__ T As_re[BK][BM + PAD], As_im[BK][BM + PAD];
-  __shared__ T Bs_re[BK][BN + PAD], Bs_im[BK][BN + PAD];
+  constexpr int LS = BM + PAD;
+  // [4 planes: A re, A im, B re, B im][KG][BK][LS]
+  __shared__ T smem[4 * KG * BK * LS];
+  static_assert(KG == 1 || 4 * KG * BK * LS >= 4 * 32 * 64, "partial-tile exchange fits the operand LDS");
+  auto lds = [&](int plane, int g, int k, int x) -> T& {
+    return smem[((plane * KG + g) * BK + k) * LS + x];
+  };
 
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x & 255;  // thread within its K-group
+  const int grp = threadIdx.x >> 8;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = (wave >> 1) * 32;
@@ -125,27 +137,31 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmDesc d, const cplx<T>* __
     }
   };
 
-  if (k_begin < k_end) load_tile(k_begin);
-  for (int k0 = k_begin; k0 < k_end; k0 += BK) {
-    // ---- registers -> LDS (re/im planes), then prefetch the next tile
+  // tiles beyond k_end load as zeros (no memory access), so every group can
+  // run the same number of iterations
+  load_tile(k_begin + grp * BK);
+  for (int k0 = k_begin; k0 < k_end; k0 += KG * BK) {
+    const int kt = k0 + grp * BK;
+    // ---- registers -> LDS (re/im planes), then prefetch this group's next tile
 #pragma unroll
     for (int r = 0; r < LOADS; ++r) {
       const int e = tid + 256 * r;
       const int ml = A_MCONT ? (e % BM) : (e / BK);
       const int kl = A_MCONT ? (e / BM) : (e % BK);
-      As_re[kl][ml] = pa[r].re;
-      As_im[kl][ml] = sa * pa[r].im;
+      lds(0, grp, kl, ml) = pa[r].re;
+      lds(1, grp, kl, ml) = sa * pa[r].im;
     }
 #pragma unroll
     for (int r = 0; r < LOADS; ++r) {
       const int e = tid + 256 * r;
       const int nl = B_NCONT ? (e % BN) : (e / BK);
       const int kl = B_NCONT ? (e / BN) : (e % BK);
-      Bs_re[kl][nl] = pb[r].re;
-      Bs_im[kl][nl] = sb * pb[r].im;
+      lds(2, grp, kl, nl) = pb[r].re;
+      lds(3, grp, kl, nl) = sb * pb[r].im;
     }
     __syncthreads();
-    if (k0 + BK < k_end) load_tile(k0 + BK);
+    if (kt + KG * BK < k_end) load_tile(kt + KG * BK);
+    else if (k0 + KG * BK < k_end) load_tile(k_end);  // this group idles in the last round: zeros
 
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
@@ -154,10 +170,10 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmDesc d, const cplx<T>* __
       T ar[2], ai[2], br[2], bi[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        ar[i] = As_re[kr][wm + 16 * i + c];
-        ai[i] = As_im[kr][wm + 16 * i + c];
-        br[i] = Bs_re[kr][wn + 16 * i + c];
-        bi[i] = Bs_im[kr][wn + 16 * i + c];
+        ar[i] = lds(0, grp, kr, wm + 16 * i + c);
+        ai[i] = lds(1, grp, kr, wm + 16 * i + c);
+        br[i] = lds(2, grp, kr, wn + 16 * i + c);
+        bi[i] = lds(3, grp, kr, wn + 16 * i + c);
       }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
@@ -177,6 +193,32 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmDesc d, const cplx<T>* __
     __syncthreads();
   }
 
+  if constexpr (KG == 2) {
+    // group 1 hands its partial tile to group 0 through the (now free) operand LDS
+    T* xs = smem + (wave * 32) * 64 + lane;  // [wave][value][lane]
+    if (grp == 1) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            xs[(((i * 2 + j) * 4 + r) * 2 + 0) * 64] = acc_re[i][j][r];
+            xs[(((i * 2 + j) * 4 + r) * 2 + 1) * 64] = acc_im[i][j][r];
+          }
+    }
+    __syncthreads();
+    if (grp == 1) return;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          acc_re[i][j][r] += xs[(((i * 2 + j) * 4 + r) * 2 + 0) * 64];
+          acc_im[i][j][r] += xs[(((i * 2 + j) * 4 + r) * 2 + 1) * 64];
+        }
+  }
   // ---- epilogue: column lane%16, rows per Mfma<T>::row of each 16x16 tile
   const int col = lane & 15;
   const T al_re = T(d.alpha_re), al_im = T(d.alpha_im);
@@ -269,8 +311,17 @@ void gemm_launch(const GemmDesc& d, const cplx<T>* A, const cplx<T>* B, cplx<T>*
   QCA_CHECK(batch * splits <= 65535, "GEMM batch*splits exceeds grid z limit");
   dim3 grid(tn, tm, batch * splits);
   const bool am = (d.a_m == 1), bn = (d.b_n == 1);
-#define QCA_GEMM_L(AM, BNC) \
-  hipLaunchKernelGGL((gemm_kernel<T, AM, BNC>), grid, dim3(256), 0, s, d, A, B, C, ws, splits, kps)
+  // two K-groups per workgroup when each has at least two K-tiles to take
+  const bool kg2 = kps >= 4 * BK;
+#define QCA_GEMM_L(AM, BNC)                                                                     \
+  do {                                                                                         \
+    if (kg2)                                                                                   \
+      hipLaunchKernelGGL((gemm_kernel<T, AM, BNC, 2>), grid, dim3(512), 0, s, d, A, B, C, ws,  \
+                         splits, kps);                                                         \
+    else                                                                                       \
+      hipLaunchKernelGGL((gemm_kernel<T, AM, BNC, 1>), grid, dim3(256), 0, s, d, A, B, C, ws,  \
+                         splits, kps);                                                         \
+  } while (0)
   if (am && bn) QCA_GEMM_L(true, true);
   else if (am) QCA_GEMM_L(true, false);
   else if (bn) QCA_GEMM_L(false, true);
