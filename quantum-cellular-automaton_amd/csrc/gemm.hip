@@ -163,6 +163,9 @@ __global__ __launch_bounds__(256 * KG) void gemm_kernel(GemmDesc d, const cplx<T
     if (kt + KG * BK < k_end) load_tile(kt + KG * BK);
     else if (k0 + KG * BK < k_end) load_tile(k_end);  // this group idles in the last round: zeros
 
+    // a group without a tile in the last round skips its MFMAs (its registers
+    // hold zeros; the barriers stay outside)
+    if (kt < k_end)
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const int kr = kk + (lane >> 4);
