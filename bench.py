@@ -108,7 +108,9 @@ def batch_bench(torch, a):
     """Aggregate 2TDVP site-updates/s of the first `a.batch` instances of the
     batched workload (mixed precision), advanced one step at a time by
     `a.batch_threads` native workers, one stream per instance; the same engines
-    stepped one after another on one host thread give the sequential rate."""
+    stepped one after another on one host thread give the sequential rate, and
+    plain FP64 engines started from the same warmed states, stepped one after
+    another, the rate of the plain path."""
     import qca_native as qn
     import workloads
     insts = workloads.batch_instances(256, seed=4)[:a.batch]
@@ -137,7 +139,23 @@ def batch_bench(torch, a):
         su = sum(e.stats()["site_updates"] for e in engines)
         rates[label] = su / dt
         log(f"batched workload: {label} {su} site-updates in {dt:.2f} s")
-    for e in engines:
+    # the plain FP64 path on the same warmed states, one instance at a time
+    plain = [qn.Engine(workloads.mpo_tensors(i), e.tensors(), algorithm=2, precision=0,
+                       chi_max=i.chi_max, dt=i.step_size, eps=i.svd_epsilon)
+             for i, e in zip(insts, engines)]
+    for e in plain:
+        e.set_fused_krylov_limit(1024)
+        e.reset_stats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e in plain:
+        e.step(a.batch_steps)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    su = sum(e.stats()["site_updates"] for e in plain)
+    rates["fp64_sequential"] = su / dt
+    log(f"batched workload: plain FP64 one after another {su} site-updates in {dt:.2f} s")
+    for e in engines + plain:
         e.close()
     return {"workload": f"first {len(insts)} instances of the batched 2TDVP draw (default_rng(4)): "
                         "N~U{15..61}, chi_max~{16,32,64,128}, d~U{1,2}, 1<=a<=b<=2d, p=0.3, dt 0.1, "
@@ -147,7 +165,9 @@ def batch_bench(torch, a):
             "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
             "one_launch_krylov_limit": {"sequential": 1024, "batched": 4096},
             "site_updates_per_s": rates["batched"], "sequential_site_updates_per_s": rates["sequential"],
-            "speedup_over_sequential": rates["batched"] / rates["sequential"]}
+            "speedup_over_sequential": rates["batched"] / rates["sequential"],
+            "fp64_sequential_site_updates_per_s": rates["fp64_sequential"],
+            "speedup_over_fp64_sequential": rates["batched"] / rates["fp64_sequential"]}
 
 
 def tdvp1_bench(torch, a):
