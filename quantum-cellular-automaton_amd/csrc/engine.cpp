@@ -166,6 +166,15 @@ Engine::Engine(int device, const EngineParams& p, const std::vector<const c128*>
     jw_.pre = dmalloc<c128>((long)jw_.pre_elems);
   }
   QCA_HIP(hipHostMalloc(&jw_.host, 64));
+  if (mixed) {
+    jw_.tol = MIXED_SVD_TOL;
+    if (const char* env = std::getenv("QCA_MIXED_SVD_TOL")) {
+      char* end = nullptr;
+      const double v = std::strtod(env, &end);
+      QCA_CHECK(end != env && *end == '\0' && v >= 0 && v < 1, "QCA_MIXED_SVD_TOL must be in [0, 1)");
+      jw_.tol = v;
+    }
+  }
   QCA_HIP(hipHostMalloc((void**)&svdres_host_, sizeof(SvdResult)));
   QCA_HIP(hipHostMalloc((void**)&ctrl_host_, sizeof(LanczosCtrl)));
   QCA_HIP(hipEventCreate(&ev0_));

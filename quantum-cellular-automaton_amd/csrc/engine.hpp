@@ -13,6 +13,11 @@ enum Precision { PREC_FP64 = 0, PREC_MIXED = 1 };
 // Default size limit of the one-launch Krylov exponential (elements).
 constexpr long FUSED_KRYLOV_DEFAULT_N = 1024;
 
+// Jacobi rotation threshold of the mixed path's SVD, relative to the column
+// norms (|g_ij| <= tol * sqrt(g_ii g_jj) leaves a pair alone).  Its complex64
+// contractions carry ~1e-7 relative error; the plain path keeps eps*sqrt(rows).
+constexpr double MIXED_SVD_TOL = 1e-10;
+
 struct EngineParams {
   int ncells = 0;
   int algorithm = 2;       // 1 = 1TDVP, 2 = 2TDVP, 3 = adaptive 1TDVP

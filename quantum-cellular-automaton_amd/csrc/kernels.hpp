@@ -163,6 +163,8 @@ struct JacobiWork {
   c128* pre = nullptr;
   size_t pre_elems = 0;
   int pred_sweeps_pre = 6;
+  // rotation threshold relative to the column norms; 0: eps * sqrt(rows)
+  double tol = 0;
 };
 
 struct JacobiLayout {

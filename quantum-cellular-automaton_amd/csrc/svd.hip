@@ -841,7 +841,7 @@ JacobiCtrl* gram_jacobi(c128* Y, int rows, int cols, c128* Z, int zrows, hipStre
   const JacobiGeom g{rows, cols, l.bs, l.nb, l.kspl};
   const JacobiGeom gz{zrows, cols, l.bs, l.nb, Z ? ceil_div(zrows, JAC_RC) : 0};
   const int np = l.nb / 2;
-  const double tol = 1.1102230246251565e-16 * sqrt((double)rows);
+  const double tol = w.tol > 0 ? w.tol : 1.1102230246251565e-16 * sqrt((double)rows);
   hipLaunchKernelGGL(jac_init, dim3(1), dim3(JAC_NT), 0, s, Y, (long)rows * cols, ctrl);
   // launch the sweeps predicted from the previous call, then read the device
   // flag once per chunk (rounds after convergence are no-ops)
@@ -890,7 +890,7 @@ void launch_jacobi_svd(c128* X, int rows, int cols, double* sigma, int* perm, Sv
   QCA_CHECK(cols >= 1 && cols <= JAC_MAXCOLS, "SVD cols out of range (1..512)");
   QCA_CHECK(w.ws != nullptr && w.host != nullptr, "Jacobi workspace not set");
   if (cols <= JAC_LMAX) {
-    const double tol = 1.1102230246251565e-16 * sqrt((double)rows);
+    const double tol = w.tol > 0 ? w.tol : 1.1102230246251565e-16 * sqrt((double)rows);
     hipLaunchKernelGGL(jac_small, dim3(1), dim3(JAC_NT), 0, s, X, rows, cols, sigma, perm, res, chi_max,
                        eps, rank_cap, tol);
     QCA_HIP(hipGetLastError());
