@@ -158,7 +158,10 @@ def test_krylov_expm(gpu, prec, kind, chil, chir, delta):
                                                    # full and ragged (34..48 columns)
                                                    (32, 32, 64, 1e-10), (24, 20, 64, 1e-10),
                                                    (17, 20, 64, 1e-10),
-                                                   (64, 64, 128, 1e-10), (128, 128, 128, 1e-10)])
+                                                   (64, 64, 128, 1e-10), (128, 128, 128, 1e-10),
+                                                   # the widest split (512 columns) and a
+                                                   # ragged 400 x 272 one capped at 128
+                                                   (256, 256, 256, 1e-10), (200, 136, 128, 1e-10)])
 def test_svd_split(gpu, right, chil, chir, chi_max, eps):
     import torch
     rng = np.random.default_rng(chil * 3 + chir + right)
@@ -194,7 +197,10 @@ def test_svd_split(gpu, right, chil, chir, chi_max, eps):
         I = np.einsum("pak,pal->kl", Al.conj(), Al)
     else:
         I = np.einsum("pkc,plc->kl", Ar, Ar.conj())
-    assert np.abs(I - np.eye(kk)).max() < 1e-13
+    # Householder products and accumulated rotations lose orthogonality like
+    # n * eps in the isometry's length n (512 at the widest split)
+    n = 2 * chil if right else 2 * chir
+    assert np.abs(I - np.eye(kk)).max() < max(1e-13, 2 * n * np.finfo(float).eps)
 
 
 @pytest.mark.parametrize("side", [0, 1])
