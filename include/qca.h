@@ -29,7 +29,10 @@
  *    overlap on one device race.
  *  - precision: 0 = complex128 contractions (plain FP64 path),
  *               1 = complex64 contractions on FP32 matrix cores (mixed path;
- *                   Krylov vectors, SVD and QR stay FP64).
+ *                   Krylov vectors, SVD and QR stay FP64).  An engine of the
+ *                   mixed path stops its Jacobi SVD at a relative off-diagonal
+ *                   of 1e-10 (plain path: eps * sqrt(rows)); the environment
+ *                   variable QCA_MIXED_SVD_TOL, read at creation, overrides it.
  */
 #ifndef QCA_H
 #define QCA_H
@@ -69,7 +72,8 @@ typedef struct qca_engine_params {
   int reserved;
   double dt;         /* reference args.step_size; propagator exp(-i pi/2 dt H) */
   double eps;        /* reference args.svd_epsilon (2TDVP truncation) */
-  double krylov_tol; /* Krylov a-posteriori error target (relative to |x|) */
+  double krylov_tol; /* Krylov a-posteriori error target (relative to |x|);
+                        0: 1e-14 (the Python drop-in passes 1e-10 for mixed) */
 } qca_engine_params;
 
 typedef struct qca_stats {

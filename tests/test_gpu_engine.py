@@ -46,6 +46,12 @@ CASES = [
     # blocked QR in 1TDVP, both Krylov paths along the chain
     ("2tdvp", 14, 2, range(2, 5), "triple", 0.1, 10, 64, 1e-10),
     ("1tdvp", 14, 2, range(2, 4), "padded", 0.05, 6, 64, 0.0),
+    # the shortest chains: one or two bonds, sweeps of a single block
+    ("2tdvp", 2, 1, range(1, 2), "single", 0.1, 6, 4, 1e-10),
+    ("2tdvp", 3, 2, range(1, 3), "triple", 0.1, 6, 4, 1e-10),
+    ("1tdvp", 2, 1, range(1, 2), "padded", 0.1, 6, 4, 0.0),
+    ("1tdvp", 3, 1, range(1, 2), "padded", 0.1, 6, 4, 0.0),
+    ("a1tdvp", 3, 1, range(1, 2), "single", 0.1, 5, 4, 1e-10),
 ]
 
 ALG_CODE = {"1tdvp": 1, "2tdvp": 2, "a1tdvp": 3}
