@@ -318,8 +318,9 @@ def main() -> int:
                          "(the command the kernel profiles are taken with)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU: every box in this pool has a single MI355X, and the "
-              "2TDVP sweep of one instance does not shard; run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU (one MI355X): the 2TDVP sweep of one instance does not "
+              "shard, and batched instances would be placed one per GPU; run with --gpus 1",
+              file=sys.stderr)
         return 2
     if a.batch_only and not a.batch:
         a.batch = 32

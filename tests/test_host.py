@@ -185,3 +185,17 @@ def test_server_rejects_oversized_and_reports_errors():
     srv = serve.Server(Boom, workers=2)
     with pytest.raises(RuntimeError, match="device lost"):
         srv.run([(0.0, serve.Request(i, s, 20, 0.0)) for i, s in enumerate(small)])
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 exits with status 2 and a
+    message, prints no result line and touches no device."""
+    import pathlib
+    import subprocess
+    import sys
+    root = pathlib.Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], cwd=root,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and "--gpus 1" in r.stderr
+    assert r.stdout == ""
