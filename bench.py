@@ -401,7 +401,8 @@ def main() -> int:
         log(f"cpu baseline ({a.cpu_seconds:.0f} s budget)")
         cpu = cpu_baseline(engines["fp64"].tensors(), a.cpu_seconds)
     out = {
-        "metric": "2TDVP site-updates/s at chi=128, mixed vs FP64 @max|d<n>|<=1e-6; p50/p99 latency",
+        "metric": "2TDVP site-updates/s at \u03c7=128, mixed vs FP64 @max|\u0394\u27e8n\u27e9|\u22641e-6; "
+                  "p50/p99 latency",
         "value": rate["mixed"],
         "unit": "site-updates/s",
         "n_gpus": 1,
