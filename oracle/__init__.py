@@ -30,7 +30,7 @@ Gauge modes of ``tdvp.TDVPOracle``:
   projector-splitting integrator; at full bond dimension it reproduces exact
   evolution.  The GPU implementation follows this mode.
 
-Parity pins (what checks each function; see tests/test_oracle_*.py):
+Parity pins (what checks each function; all in tests/test_oracle.py):
 
 * automaton: golden MPO tensors from the reference for every (d, a, b), and the
   closed-form Hamiltonian vs the MPO contracted to a dense matrix (N <= 9).
@@ -42,5 +42,10 @@ Parity pins (what checks each function; see tests/test_oracle_*.py):
 * tdvp reference mode: golden trajectories of the reference (full-chi N=8 runs,
   short truncating runs, and config 1 in full).
 * tdvp consistent mode: step-1 rows of every golden trajectory (both modes agree
-  on the first step from a product state), and full-rank runs vs exact.
+  on the first step from a product state), and full-rank runs vs exact
+  (2, 3 and 7 cells, all three algorithms).
+* tdvp adaptive targets (a1tdvp): golden reference runs replayed in reference
+  mode, bond targets included.
+* planted mistakes (transposed MPO indices, wrong propagator sign, wrong
+  neighbour window) fail these pins.
 """
