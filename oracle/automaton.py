@@ -36,6 +36,8 @@ def automaton_states(distance: int, interval: range):
     edges: list[tuple[int, int, np.ndarray]] = []
 
     def visit(k0: int, k1: int, s: bool) -> int:
+        """Index of state (k0, k1, s), inserted depth first on first visit
+        (reference tensor_networks/mpo.py:90-150, find_or_insert_recursive)."""
         key = (k0, k1, s)
         if key in states:
             return states.index(key)
@@ -117,7 +119,11 @@ def mpo_dense(Ws: list[np.ndarray]) -> np.ndarray:
 
 
 def neighbour_masks(ncells: int, distance: int) -> list[int]:
-    """Bit mask of the neighbours of every cell; cell i is bit (N-1-i)."""
+    """Bit mask of the neighbours of every cell; cell i is bit (N-1-i).
+
+    Neighbours are the cells within ``distance`` on the open chain
+    (reference parameters/rules.py:2-6, the rule the MPO builder
+    tensor_networks/mpo.py:159-200 encodes)."""
     masks = []
     for i in range(ncells):
         m = 0
@@ -130,6 +136,10 @@ def neighbour_masks(ncells: int, distance: int) -> list[int]:
 
 def apply_h(psi: np.ndarray, ncells: int, distance: int, interval: range) -> np.ndarray:
     """Closed-form (H psi)[x] = sum_i [popcount(x & nbr_i) in interval] psi[x ^ bit_i].
+
+    The operator the rule MPO of reference tensor_networks/mpo.py:159-200
+    encodes, which the reference's exact path densifies with MPO.as_matrix
+    (mpo.py:219-228, algorithms/exact.py:17).
 
     The flip condition does not depend on bit i itself, so H is real symmetric.
     Vectorised over the 2^N basis (cheap for N <= ~24 on the host).
@@ -157,6 +167,7 @@ def _popcount(v: np.ndarray) -> np.ndarray:
 
 
 def dense_h(ncells: int, distance: int, interval: range) -> np.ndarray:
-    """Dense closed-form Hamiltonian (small N only)."""
+    """Dense closed-form Hamiltonian (small N only): the reference's
+    MPO.as_matrix (tensor_networks/mpo.py:219-228) without the MPO."""
     dim = 1 << ncells
     return apply_h(np.eye(dim, dtype=complex), ncells, distance, interval)

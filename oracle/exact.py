@@ -19,6 +19,8 @@ from . import mps as M
 
 
 class ExactOracle:
+    """State-vector evolution of reference algorithms/exact.py:9-28."""
+
     def __init__(self, As, ncells: int, distance: int, interval: range, dt: float,
                  mode: str = "dense", tol: float = 1e-14):
         self.n, self.d, self.interval = ncells, distance, interval
@@ -34,6 +36,7 @@ class ExactOracle:
             raise ValueError(mode)
 
     def do_time_step(self) -> None:
+        """psi <- exp(-i (pi/2) dt H) psi (reference algorithms/exact.py:27-28)."""
         if self.mode == "dense":
             self.psi = self.U @ self.psi
         else:
@@ -42,7 +45,9 @@ class ExactOracle:
                 tol=self.tol, max_iter=200)
 
     def populations(self) -> np.ndarray:
-        """<n_i> = sum_x |psi_x|^2 x_i (marginals of |psi|^2)."""
+        """<n_i> = sum_x |psi_x|^2 x_i (marginals of |psi|^2): the population
+        MPS.measure (reference tensor_networks/mps.py:100-132) reads from
+        MPS.from_vector(psi) (algorithms/exact.py:20-21)."""
         p = np.abs(self.psi) ** 2
         x = np.arange(1 << self.n)
         return np.array([p[(x >> (self.n - 1 - i)) & 1 == 1].sum() for i in range(self.n)])

@@ -37,13 +37,17 @@ def merge_w(Wl: np.ndarray, Wr: np.ndarray) -> np.ndarray:
 
 
 def heff_apply(x: np.ndarray, L: np.ndarray, R: np.ndarray, W: np.ndarray) -> np.ndarray:
-    """Matrix-free effective Hamiltonian (1-site or 2-site by the size of W)."""
+    """Matrix-free effective Hamiltonian (1-site or 2-site by the size of W):
+    the operator reference algorithms/tdvp.py:298-311 (_assemble_H_eff)
+    assembles and tdvp.py:350-365 (_evolve_A) applies, never formed densely."""
     t = np.einsum("awx,pac->pwxc", L, x, optimize=True)          # L first
     t = np.einsum("pqwv,pwxc->qvxc", W, t, optimize=True)        # then W
     return np.einsum("qvxc,cvy->qxy", t, R, optimize=True)       # then R
 
 
 def bond_apply(C: np.ndarray, L: np.ndarray, R: np.ndarray) -> np.ndarray:
+    """Bond effective Hamiltonian applied to C, matrix-free (reference
+    algorithms/tdvp.py:312-326, _assemble_K_eff, used by _evolve_bond 164-188)."""
     return np.einsum("awx,bwy,ab->xy", L, R, C, optimize=True)
 
 
@@ -89,6 +93,9 @@ def expm_dense(Hmat: np.ndarray, v: np.ndarray, delta: float) -> np.ndarray:
 def expm_krylov(apply, v: np.ndarray, delta: float, tol: float = 1e-14,
                 max_iter: int = 80):
     """exp(-i (pi/2) delta H) v by Lanczos with full reorthogonalisation (FP64).
+
+    The propagator of reference lautils/lautils.py:44-81 (calculate_U and
+    timestep form exp(-i (pi/2) delta H) densely); here applied in a Krylov space.
 
     ``apply`` computes H x for flattened x.  Stops when the a-posteriori error
     beta_0 * beta_m * |[exp(-i tau T_m)]_{m,0}| drops below ``tol`` * |v|, or on

@@ -79,6 +79,8 @@ def make_site_canonical(As: list, i: int) -> None:
 
 
 def bond_dims(As: list) -> np.ndarray:
+    """Bond dimensions as MPS.measure records them (reference
+    tensor_networks/mps.py:106-109)."""
     return np.array([A.shape[1] for A in As] + [As[-1].shape[2]], dtype=float)
 
 
@@ -114,7 +116,8 @@ def measure_reference(As: list):
 
 
 def measure_consistent(As: list):
-    """Same observables without moving the orthogonality centre.
+    """Same observables as reference tensor_networks/mps.py:100-140 without
+    moving the orthogonality centre.
 
     Requires the TDVP invariant that sites 1..N-1 are right-isometric (centre
     at site 0): then rho_i = sum E_{i-1}[a,a'] A_i[p,a,b] conj(A_i[p',a',b])

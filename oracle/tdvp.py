@@ -16,6 +16,10 @@ ONE = np.ones((1, 1, 1), dtype=complex)
 
 
 class TDVPOracle:
+    """1-site, 2-site and adaptive 1-site TDVP of reference
+    algorithms/tdvp.py:10-63 (construction, do_time_step) and its sweeps
+    (tdvp.py:65-145), in the reference's or the gauge-consistent operation order."""
+
     def __init__(self, As, Ws, algorithm: str, dt: float, chi_max: int, eps: float,
                  gauge: str = "reference", expm: str = "dense", krylov_tol: float = 1e-14):
         if algorithm not in ("1tdvp", "2tdvp", "a1tdvp"):
