@@ -15,7 +15,7 @@ from parameters import Rules
 
 def entropy_bits(rho: np.ndarray) -> float:
     """-Tr(rho log2 rho) of a 2x2 density matrix from its eigenvalues
-    (0 log 0 = 0)."""
+    (0 log 0 = 0); the reference takes logm of rho (tensor_networks/mps.py:135-140)."""
     w = np.linalg.eigvalsh(0.5 * (rho + rho.conj().T))
     w = w[w > 0]
     return float(-(w * np.log2(w)).sum())
@@ -35,28 +35,31 @@ def write_observables(rho: np.ndarray, bond_dims_now, population, d_population,
 
 
 class Algorithm(ABC):
+    """Time-evolution backend (reference algorithms/algorithm.py:10-63)."""
+
     _H = None
     args = None
 
     @abstractmethod
     def __init__(self, psi_0, H, args) -> None:
+        """Initial state, rule MPO and parsed arguments (reference algorithm.py:14-18)."""
         self.psi = psi_0
         self._H = H
         self.args = args
 
     @abstractmethod
     def do_time_step(self) -> None:
-        pass
+        """Advance the state by one step (reference algorithm.py:20-22)."""
 
     @property
     @abstractmethod
     def psi(self):
-        pass
+        """The current state as an MPS (reference algorithm.py:24-27)."""
 
     @psi.setter
     @abstractmethod
     def psi(self, value) -> None:
-        pass
+        """Replace the current state (reference algorithm.py:29-32)."""
 
     @classmethod
     def classical_evolution(cls, first_column: np.ndarray, rules: Rules, plot_steps: int) -> np.ndarray:
@@ -75,4 +78,5 @@ class Algorithm(ABC):
 
     @abstractmethod
     def measure(self, population, d_population, single_site_entropy, bond_dims) -> None:
-        pass
+        """Write <n_i>, its rounding, the single-site entropy and the bond
+        dimensions into the given rows (reference algorithm.py:58-63)."""

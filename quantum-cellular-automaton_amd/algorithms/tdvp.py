@@ -39,7 +39,14 @@ def _precision_code(args) -> int:
 
 
 class TDVP(Algorithm):
+    """1-site, 2-site and adaptive 1-site TDVP (reference algorithms/tdvp.py:10-63)
+    on one device-resident engine."""
+
     def __init__(self, psi_0, H, args) -> None:
+        """Canonicalises psi_0 and builds the right environments on the device
+        (reference tdvp.py:12-41).  ``args.algorithm`` picks the sweep; the
+        additive ``precision``, ``krylov_tol`` and ``device`` default to FP64,
+        1e-14 (1e-10 mixed) and 0."""
         self._engine = None
         super().__init__(psi_0, H, args)
         algo = getattr(args, "algorithm", "2tdvp")
@@ -53,6 +60,7 @@ class TDVP(Algorithm):
             # the mixed path's complex64 matvecs are accurate to ~1e-7: a Krylov
             # tolerance three orders below that buys nothing
             tol = 1e-14 if prec == 0 else 1e-10
+        self._tol = tol
         self._engine = Engine(H.W, self._psi.A, algorithm=self._algo, precision=prec,
                               chi_max=int(args.max_bond_dim), dt=float(args.step_size),
                               eps=float(args.svd_epsilon), krylov_tol=tol,
@@ -61,6 +69,8 @@ class TDVP(Algorithm):
 
     @property
     def psi(self):
+        """The caller's MPS, its tensors refreshed from the device when stale
+        (reference tdvp.py:42-44 returns the MPS it evolves in place)."""
         if self._engine is not None and not self._host_fresh:
             self._psi.A[:] = self._engine.tensors()
             self._host_fresh = True
@@ -68,6 +78,8 @@ class TDVP(Algorithm):
 
     @psi.setter
     def psi(self, value) -> None:
+        """A new state (reference tdvp.py:46-48): the device instance is rebuilt
+        with the same algorithm, precision and tolerances."""
         self._psi = value
         if getattr(self, "_engine", None) is not None:
             # a new state: rebuild the device instance (canonicalise + environments)
@@ -76,15 +88,17 @@ class TDVP(Algorithm):
                                   precision=_precision_code(self.args),
                                   chi_max=int(self.args.max_bond_dim),
                                   dt=float(self.args.step_size),
-                                  eps=float(self.args.svd_epsilon),
+                                  eps=float(self.args.svd_epsilon), krylov_tol=self._tol,
                                   device=int(getattr(self.args, "device", 0) or 0))
             self._host_fresh = False
 
     @property
     def engine(self) -> Engine:
+        """The native engine (qca_engine_* in include/qca.h)."""
         return self._engine
 
     def do_time_step(self) -> None:
+        """One time step: right sweep then left sweep (reference tdvp.py:50-63)."""
         self._engine.step(1)
         self._host_fresh = False
 
@@ -94,4 +108,5 @@ class TDVP(Algorithm):
         self._engine.observables(population, d_population, single_site_entropy, bond_dims)
 
     def populations(self) -> np.ndarray:
+        """<n_i> = Re rho_i[1][1] (reference tensor_networks/mps.py:118-131)."""
         return self._engine.rho()[:, 1, 1].real.copy()

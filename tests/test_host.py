@@ -199,3 +199,27 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     assert "one GPU" in r.stderr and "--gpus 1" in r.stderr
     assert r.stdout == ""
+
+
+def test_tdvp_new_state_keeps_precision_and_tolerances(monkeypatch):
+    """Assigning TDVP.psi rebuilds the device engine with the algorithm,
+    precision and Krylov tolerance chosen at construction (the engine class
+    is replaced by a recorder, so no device is needed)."""
+    from types import SimpleNamespace
+    import algorithms.tdvp as T
+    made = []
+
+    class Recorder:
+        def __init__(self, W, A, **kw):
+            made.append(kw)
+
+        def close(self):
+            pass
+
+    monkeypatch.setattr(T, "Engine", Recorder)
+    args = SimpleNamespace(algorithm="2tdvp", precision="mixed", krylov_tol=0.0, max_bond_dim=8,
+                           step_size=0.1, svd_epsilon=1e-10, device=0)
+    t = T.TDVP(SimpleNamespace(A=[None]), SimpleNamespace(W=[None]), args)
+    t.psi = SimpleNamespace(A=[None])
+    assert len(made) == 2 and made[0] == made[1]
+    assert made[0]["krylov_tol"] == 1e-10 and made[0]["precision"] == 1 and made[0]["algorithm"] == 2
