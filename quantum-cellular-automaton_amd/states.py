@@ -21,6 +21,7 @@ def _product(plist) -> MPS:
 
 
 def blinker(ncells: int, width: int = 1) -> MPS:
+    """Cells mid-width and mid+width alive (reference states.py:11-15)."""
     p = [0.0] * ncells
     mid = int(ncells / 2)
     p[mid - width] = p[mid + width] = 1.0
@@ -28,6 +29,7 @@ def blinker(ncells: int, width: int = 1) -> MPS:
 
 
 def triple_blinker(ncells: int) -> MPS:
+    """Cells mid-2, mid and mid+2 alive (reference states.py:18-22)."""
     p = [0.0] * ncells
     mid = int(ncells / 2)
     p[mid - 2] = p[mid] = p[mid + 2] = 1.0
@@ -43,56 +45,69 @@ def three_adjacent(ncells: int) -> MPS:
 
 
 def full_blinker(ncells: int) -> MPS:
+    """Every odd cell alive (reference states.py:25-31)."""
     return _product([float(i % 2) for i in range(ncells)])
 
 
 def single(ncells: int, position: int | None = None) -> MPS:
+    """One alive cell, the middle one by default (reference states.py:34-37)."""
     p = [0.0] * ncells
     p[int(ncells / 2) if position is None else position] = 1.0
     return _product(p)
 
 
 def single_bottom(ncells: int) -> MPS:
+    """Cell 0 alive (reference states.py:40-41)."""
     return single(ncells, 0)
 
 
 def all_ket_0(ncells: int) -> MPS:
+    """Every cell dead (reference states.py:44-46)."""
     return _product([0.0] * ncells)
 
 
 def all_ket_1(ncells: int) -> MPS:
+    """Every cell alive (reference states.py:49-51)."""
     return _product([1.0] * ncells)
 
 
 def only_outer(ncells: int) -> MPS:
+    """The two end cells alive (reference states.py:54-57)."""
     p = [0.0] * ncells
     p[0] = p[-1] = 1.0
     return _product(p)
 
 
 def all_ket_1_but_outer(ncells: int, distance: int) -> MPS:
+    """All alive except `distance` dead cells at each end (reference states.py:60-64)."""
     return _product([0.0] * distance + [1.0] * (ncells - 2 * distance) + [0.0] * distance)
 
 
 def equal_superposition(ncells: int) -> MPS:
+    """p = 1/2 on every cell (reference states.py:67-69)."""
     return _product([0.5] * ncells)
 
 
 def equal_superposition_but_outer(ncells: int, distance: int) -> MPS:
+    """p = 1/2 except `distance` dead cells at each end (reference states.py:72-76)."""
     return _product([0.0] * distance + [0.5] * (ncells - 2 * distance) + [0.0] * distance)
 
 
 def gradient(ncells: int, reverse: bool = False) -> MPS:
+    """p_i = sin(pi i / (N-1) / 2), optionally reversed (reference states.py:79-83)."""
     p = [np.sin(np.pi * i / (ncells - 1) / 2) for i in range(ncells)]
     return _product(list(reversed(p)) if reverse else p)
 
 
 def rand_plist(ncells: int, p: float = 0.5, seed: int = 0) -> list:
+    """Each cell alive with probability p, from a seeded generator (the reference's
+    states.py:86-88 draws from the unseeded `random` module)."""
     rng = np.random.default_rng(seed)
     return [1.0 if x < p else 0.0 for x in rng.random(ncells)]
 
 
 def rand(ncells: int, p: float = 0.5, seed: int = 0) -> MPS:
+    """Product state of rand_plist (reference states.py:86-88)."""
     return _product(rand_plist(ncells, p, seed))
 
 

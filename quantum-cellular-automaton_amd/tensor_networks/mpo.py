@@ -45,6 +45,8 @@ def _automaton(distance: int, interval: range):
 
 
 class MPO(object):
+    """Tensors W[i] of shape (2, 2, Dl, Dr) (reference tensor_networks/mpo.py:152-157)."""
+
     W: list = []
 
     def __init__(self, Wlist: list) -> None:
@@ -52,6 +54,9 @@ class MPO(object):
 
     @classmethod
     def hamiltonian_from_rules(cls, rules: Rules) -> "MPO":
+        """The rule Hamiltonian: the state automaton's transitions as the bulk
+        tensor, closed at both ends by `distance` dead cells (reference
+        mpo.py:158-200).  W tensors equal the reference's entry for entry."""
         if rules.ncells < 2:
             raise ValueError("the rule Hamiltonian needs at least two cells")
         nstates, edges = _automaton(rules.distance, rules.activation_interval)
@@ -76,12 +81,15 @@ class MPO(object):
 
     @classmethod
     def merge_mpo_tensor_pair(cls, W0, W1) -> np.ndarray:
+        """Two neighbouring tensors as one, physical indices paired (reference
+        mpo.py:202-217)."""
         W = np.einsum("abxm,cdmy->acbdxy", W0, W1)
         s = W.shape
         return W.reshape(s[0] * s[1], s[2] * s[3], s[4], s[5])
 
     def as_matrix(self) -> np.ndarray:
-        """Dense 2^N matrix, site 0 most significant (small N only)."""
+        """Dense 2^N matrix, site 0 most significant (small N only; reference
+        mpo.py:219-228)."""
         H = self.W[0]
         for W in self.W[1:]:
             H = self.merge_mpo_tensor_pair(H, W)

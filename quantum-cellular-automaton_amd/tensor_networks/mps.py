@@ -11,6 +11,8 @@ import numpy as np
 
 
 class MPS(object):
+    """Tensors A[i] of shape (2, D[i], D[i+1]) (reference tensor_networks/mps.py:14-25)."""
+
     A: list = []
 
     def __init__(self, Alist: list) -> None:
@@ -18,6 +20,7 @@ class MPS(object):
 
     @classmethod
     def from_tensors(cls, Alist) -> "MPS":
+        """Complex copies of the given tensors (reference mps.py:27-32)."""
         return cls(Alist=[np.array(A, dtype=complex) for A in Alist])
 
     @classmethod
@@ -50,6 +53,7 @@ class MPS(object):
             return cls(Alist=[f[f"arr_{i}"] for i in range(len(f.files))])
 
     def write_to_file(self, path: str) -> None:
+        """npz with arr_0 .. arr_{N-1} (reference mps.py:142-144)."""
         with open(path, "wb") as f:
             np.savez(f, *self.A)
 
@@ -63,9 +67,11 @@ class MPS(object):
         return np.trace(psi, axis1=1, axis2=2)
 
     def bond_dims(self) -> list:
+        """The N+1 bond dimensions as measure() records them (reference mps.py:106-109)."""
         return [A.shape[1] for A in self.A] + [self.A[-1].shape[2]]
 
     def is_valid_mps(self) -> bool:
+        """Open boundaries and matching neighbour bonds (reference mps.py:216-225)."""
         if self.A[0].shape[1] != 1 or self.A[-1].shape[2] != 1:
             return False
         return all(self.A[i].shape[2] == self.A[i + 1].shape[1] for i in range(len(self.A) - 1))
