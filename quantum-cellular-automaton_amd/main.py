@@ -4,6 +4,7 @@ from quantum_game import QuantumGame
 
 
 def main() -> None:
+    """Parse the command line and run (reference main.py:5-9)."""
     QuantumGame(Parser.instance()).start()
 
 

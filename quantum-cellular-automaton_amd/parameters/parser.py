@@ -24,6 +24,8 @@ def _positive(value):
 
 
 def build_argparser() -> argparse.ArgumentParser:
+    """The reference's flags and defaults (reference parameters/parser.py:14-179),
+    plus --precision, --device and --krylov-tol."""
     p = argparse.ArgumentParser(description="A classical simulation of the quantum game of life",
                                 formatter_class=argparse.ArgumentDefaultsHelpFormatter)
     g = p.add_argument_group("Rules")
@@ -111,6 +113,7 @@ class Parser(object):
 
     @classmethod
     def instance(cls) -> "Parser":
+        """Process-wide parsed arguments (reference parser.py:212-219)."""
         try:
             return cls._instance
         except AttributeError:

@@ -26,7 +26,11 @@ from tensor_networks import MPO, MPS
 
 
 class QuantumGame(object):
+    """Runs every initial state and writes the reference's output files
+    (reference quantum_game.py:16-146)."""
+
     def __init__(self, args: Parser) -> None:
+        """Rule MPO and the named or file initial states (reference quantum_game.py:18-26)."""
         logging.basicConfig(level=logging.INFO)
         self.args = args
         self.H = MPO.hamiltonian_from_rules(args.rules)
@@ -35,15 +39,22 @@ class QuantumGame(object):
                                [MPS.from_file(f) for f in args.initial_state_files])
 
     def file_name(self, state_name: str) -> str:
+        """Output file stem (reference quantum_game.py:49-58)."""
         a = self.args
         iv = a.rules.activation_interval
         name = (f"{state_name}{a.rules.ncells}_{a.rules.distance}{iv.start}{iv.stop}_"
                 f"{str(a.step_size).replace('.', '')}_{a.algorithm}")
         if a.algorithm != "exact":
             name += f"_{a.max_bond_dim}"
+            if a.algorithm == "a1tdvp":
+                name += f"_{a.approximative_evolution_method}"
+                if a.approximative_evolution_method == "taylor":
+                    name += f"_{a.taylor_steps}"
         return name
 
     def run_state(self, index: int, initial_state, data_dir: str = "data") -> dict:
+        """One state: measure every plot step, evolve between them, write the
+        CSV and npz outputs (reference quantum_game.py:30-146)."""
         args = self.args
         n = args.rules.ncells
         rows = args.plot_steps
@@ -81,4 +92,5 @@ class QuantumGame(object):
                     bond_dims=bond_dims, name=name)
 
     def start(self, data_dir: str = "data") -> list:
+        """Every initial state in order (reference quantum_game.py:28-30)."""
         return [self.run_state(i, s, data_dir) for i, s in enumerate(self.initial_states)]

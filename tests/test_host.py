@@ -223,3 +223,25 @@ def test_tdvp_new_state_keeps_precision_and_tolerances(monkeypatch):
     t.psi = SimpleNamespace(A=[None])
     assert len(made) == 2 and made[0] == made[1]
     assert made[0]["krylov_tol"] == 1e-10 and made[0]["precision"] == 1 and made[0]["algorithm"] == 2
+
+
+def test_output_file_names_follow_the_reference():
+    """Output file stems as reference quantum_game.py:49-58 builds them,
+    including the a1tdvp method suffix."""
+    from types import SimpleNamespace
+    from quantum_game import QuantumGame
+    g = object.__new__(QuantumGame)
+
+    def args(alg, method="taylor"):
+        rules = SimpleNamespace(ncells=15, distance=1, activation_interval=range(1, 2))
+        return SimpleNamespace(rules=rules, step_size=0.1, algorithm=alg, max_bond_dim=16,
+                               approximative_evolution_method=method, taylor_steps=5)
+
+    g.args = args("2tdvp")
+    assert g.file_name("single") == "single15_112_01_2tdvp_16"
+    g.args = args("exact")
+    assert g.file_name("single") == "single15_112_01_exact"
+    g.args = args("a1tdvp")
+    assert g.file_name("single") == "single15_112_01_a1tdvp_16_taylor_5"
+    g.args = args("a1tdvp", "exact_exponential")
+    assert g.file_name("0") == "015_112_01_a1tdvp_16_exact_exponential"
