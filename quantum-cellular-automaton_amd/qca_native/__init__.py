@@ -114,6 +114,7 @@ def stream_ptr(stream=None) -> int | None:
 
 
 def device_count() -> int:
+    """HIP devices visible to the library (qca_device_count)."""
     n = c_int(0)
     check(lib().qca_device_count(ctypes.byref(n)))
     return n.value
@@ -154,6 +155,7 @@ class Engine:
         check(lib().qca_engine_set_fused_krylov_limit(self._h, int(max_n)))
 
     def close(self) -> None:
+        """Release the device instance (qca_engine_destroy)."""
         if getattr(self, "_h", None):
             check(lib().qca_engine_destroy(self._h))
             self._h = None
@@ -165,9 +167,11 @@ class Engine:
             pass
 
     def step(self, nsteps: int = 1) -> None:
+        """nsteps TDVP time steps on the device (qca_engine_step; reference tdvp.py:50-63 per step)."""
         check(lib().qca_engine_step(self._h, nsteps))
 
     def rho(self) -> np.ndarray:
+        """Single-site density matrices (N, 2, 2) (qca_engine_measure)."""
         out = np.zeros((self.n, 2, 2), dtype=np.complex128)
         check(lib().qca_engine_measure(self._h, out.ctypes.data))
         return out
@@ -184,28 +188,34 @@ class Engine:
         return tuple(bufs)
 
     def bond_dims(self) -> np.ndarray:
+        """The N+1 current bond dimensions (qca_engine_bond_dims)."""
         out = (c_int * (self.n + 1))()
         check(lib().qca_engine_bond_dims(self._h, out))
         return np.array(list(out), dtype=int)
 
     def site(self, i: int) -> np.ndarray:
+        """Copy of site tensor i, shape (2, D[i], D[i+1]) (qca_engine_get_site)."""
         bd = self.bond_dims()
         out = np.zeros((2, bd[i], bd[i + 1]), dtype=np.complex128)
         check(lib().qca_engine_get_site(self._h, i, out.ctypes.data))
         return out
 
     def tensors(self) -> list:
+        """Copies of every site tensor (the MPS the engine evolves)."""
         return [self.site(i) for i in range(self.n)]
 
     def stats(self) -> dict:
+        """Running counters and device times since creation or reset (qca_engine_stats)."""
         s = Stats()
         check(lib().qca_engine_stats(self._h, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in Stats._fields_}
 
     def reset_stats(self) -> None:
+        """Zero the counters (qca_engine_reset_stats)."""
         check(lib().qca_engine_reset_stats(self._h))
 
     def gpu_ms(self) -> float:
+        """Device time of the engine's stream since creation (qca_engine_gpu_ms)."""
         v = c_double(0)
         check(lib().qca_engine_gpu_ms(self._h, ctypes.byref(v)))
         return v.value
