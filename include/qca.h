@@ -48,9 +48,10 @@ extern "C" {
 
 /* Library version (major*10000 + minor*100 + patch). */
 int qca_version(void);
-/* Message of the last failed call on this thread ("" if none). */
+/* Message of the last failed call on this thread ("" if none).  No
+ * counterpart in the reference (error reporting of the C-ABI). */
 const char* qca_last_error(void);
-/* Number of visible HIP devices. */
+/* Number of visible HIP devices.  No counterpart in the reference. */
 int qca_device_count(int* count);
 
 /* ------------------------------------------------------------------ engine
@@ -90,6 +91,8 @@ typedef struct qca_stats {
  * tdvp.py:24-26) and builds all right environments (tdvp.py:37-39). */
 int qca_engine_create(const qca_engine_params* params, const double* const* W, const int* Wdl,
                       const int* Wdr, const double* const* A, const int* chis, qca_engine** out);
+/* Frees the engine's device state (the reference's TDVP object going out of
+ * scope, algorithms/tdvp.py:10). */
 int qca_engine_destroy(qca_engine* e);
 /* nsteps calls of reference TDVP.do_time_step (tdvp.py:50-63), gauge-consistent. */
 int qca_engine_step(qca_engine* e, int nsteps);
@@ -106,16 +109,19 @@ int qca_engine_observables(qca_engine* e, double* population, double* d_populati
                            double* single_site_entropy, double* bond_dims);
 /* Current bond dimensions, host out[N+1] (reference mps.py:106-109). */
 int qca_engine_bond_dims(qca_engine* e, int* out);
-/* Copy site tensor i to host complex128 (2, chi_i, chi_{i+1}). */
+/* Copy site tensor i to host complex128 (2, chi_i, chi_{i+1}): the tensor
+ * psi.A[i] of the reference's evolved MPS (algorithms/tdvp.py:42-44). */
 int qca_engine_get_site(qca_engine* e, int site, double* out);
 /* Counters since the last reset.  Waits for the engine's stream: the
  * one-launch Krylov exponentials keep their counts (and kernel time, in
  * krylov_ms) on the device.  No counterpart in the reference (instrumentation
  * for the site-update and flop accounting of the benchmark). */
 int qca_engine_stats(qca_engine* e, qca_stats* out);
+/* Zeroes the counters above.  No counterpart in the reference. */
 int qca_engine_reset_stats(qca_engine* e);
 /* Accumulated GPU time (ms) of the engine's stream since the last reset,
- * measured with HIP events around each qca_engine_step call. */
+ * measured with HIP events around each qca_engine_step call.  No
+ * counterpart in the reference (benchmark instrumentation). */
 int qca_engine_gpu_ms(qca_engine* e, double* ms);
 /* Krylov blocks of at most max_n elements (0..4096) use the one-launch
  * exponential (one workgroup, no host round trip); larger ones use the
@@ -193,7 +199,8 @@ int qca_exact_expm(void* psi, int ncells, int distance, int lo, int hi, double d
  * used on the exact state (reference exact.py:19-21, mps.py:54-73, 100-140). */
 int qca_exact_rho(const void* psi, int ncells, double* rho_out, void* stream);
 
-/* Release the primitives' workspace on the current device. */
+/* Release the primitives' workspace on the current device.  No counterpart
+ * in the reference (device memory management of the C-ABI). */
 int qca_release_workspace(void);
 
 #ifdef __cplusplus
