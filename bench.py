@@ -4,14 +4,24 @@
 
 Workload (the 41-cell single-instance 2TDVP case): N=41 cells, distance 2,
 activation interval [2,4] closed (Python range(2,5)), cells 19..21 alive,
-dt=0.1, chi_max=128, SVD cutoff 1e-10.  Two device-resident engines evolve the
-same state, one with complex64 contractions on FP32 matrix cores (mixed) and
-one in complex128 (plain); W untimed warmup steps grow the bonds, then EXACTLY
-K time steps of each engine are timed between synchronisations.
+dt=0.1, chi_max=128, SVD cutoff 1e-10.  Preparation (fixed, independent of
+the flags): a plain FP64 engine takes PREP_STEPS = 30 steps from the product
+state, after which 26 of the 40 bonds sit at chi_max; a mixed engine (complex64
+contractions on FP32 matrix cores) is built from that state.  Both engines then
+take W untimed warmup steps, and EXACTLY K time steps of each are timed between
+synchronisations.
 
 A site-update is one two-site block update (2(N-1) per time step).  ``value``
-is the mixed engine's site-updates/s; the FP64 rate, their ratio and the
-largest |<n_i>_mixed - <n_i>_fp64| over the timed steps are reported beside it.
+is the mixed engine's site-updates/s on the saturated state; the FP64 rate,
+their ratio, the largest |<n_i>_mixed - <n_i>_fp64| over the timed steps and
+the number of bonds at chi_max at the start and end of the timed region are
+reported beside it.
+
+``batched``: B copies of the prepared chi=128 state advanced together (mixed
+precision), against the plain FP64 single-instance rate above: the
+batched-vs-plain ratio of the workload's target.
+``served``: a window of Poisson arrivals (batched-workload draws plus 10%
+101-cell chi-256 requests) served with continuous batching; p50/p99 latency.
 
 ``roofline``: the dominant contraction, the two-site effective-Hamiltonian
 application (GEMM over the left environment -> sparse MPO mixing -> GEMM over
@@ -41,13 +51,17 @@ for p in (ROOT, PKG):
         sys.path.insert(0, p)
 
 N, DIST, IV, DT, CHI, EPS = 41, 2, range(2, 5), 0.1, 128, 1e-10
+# FP64 steps that bring the 41-cell state to chi_max (26 of 40 bonds at 128)
+# before any warmup or timed step
+PREP_STEPS = 30
 
 # HBM bytes moved by one 2-site H_eff application at chi=128, D=13, measured with
 # rocprofv3 FETCH_SIZE and WRITE_SIZE passes over `bench.py --roofline-only`
-# (the counters scaled by a 256 MiB device copy in the same runs).  The
-# algorithmic minimum is 4.46 MB (c64); the rest is the L2-spilling
-# intermediates and split-K partials.
+# (the counters scaled by a 256 MiB device copy in the same runs), recorded in
+# profiles/heff2_chi128_d13_hbm_bytes.md: profile-derived, not measured by this
+# run.  The algorithmic minimum is 4.46 MB (c64).
 HEFF2_TRAFFIC_BYTES = {"mixed": 66.63e6, "fp64": 132.68e6}
+HEFF2_TRAFFIC_SOURCE = "profiles/heff2_chi128_d13_hbm_bytes.md"
 
 
 def log(msg: str) -> None:
@@ -170,6 +184,65 @@ def batch_bench(torch, a):
             "speedup_over_fp64_sequential": rates["batched"] / rates["fp64_sequential"]}
 
 
+def batch128_bench(torch, a, tensors, fp64_rate):
+    """B copies of the prepared 41-cell chi=128 state (every instance starts
+    with 26+ of its 40 bonds at chi_max), mixed precision, advanced together
+    for `a.batch128_warmup` untimed and `a.batch128_steps` timed steps, two
+    ways: as one group in lockstep with grouped launches (qca_group_step, the
+    reported rate) and, for comparison, one stream per instance driven by
+    `a.batch_threads` native workers (qca_batch_step).  The ratio against the
+    plain FP64 single-instance rate of the headline is the batched-vs-plain
+    figure of the workload's target."""
+    import qca_native as qn
+    Ws = mpo().W
+    B = a.batch128
+    out = {"workload": f"{B} instances of the 41-cell chi=128 2TDVP case, each from the prepared "
+                       f"state ({PREP_STEPS} FP64 steps), mixed precision",
+           "instances": B, "warmup_steps": a.batch128_warmup, "timed_steps": a.batch128_steps,
+           "fp64_single_site_updates_per_s": fp64_rate}
+    for method in ("grouped", "streams"):
+        log(f"batched chi=128 ({method}): {B} instances of the prepared state")
+        engines = [qn.Engine(Ws, tensors, algorithm=2, precision=1, chi_max=CHI, dt=DT, eps=EPS,
+                             krylov_tol=a.mixed_krylov_tol, fused_krylov_max_n=4096) for _ in range(B)]
+        if method == "grouped":
+            grp = qn.Group(engines)
+            run = grp.step
+        else:
+            grp = None
+            run = lambda k: qn.batch_step(engines, k, a.batch_threads)  # noqa: E731
+        run(a.batch128_warmup)
+        for e in engines:
+            e.reset_stats()
+        bd0 = engines[0].bond_dims()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run(a.batch128_steps)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        su = sum(e.stats()["site_updates"] for e in engines)
+        bd1 = engines[0].bond_dims()
+        rate = su / dt
+        res = {"site_updates": su, "seconds": dt, "site_updates_per_s": rate,
+               "speedup_over_fp64_single": rate / fp64_rate,
+               "bonds_at_chi_max": {"start": int((bd0 == CHI).sum()), "end": int((bd1 == CHI).sum())}}
+        if grp is not None:
+            res["group_stats"] = grp.stats()
+            res["method"] = "one group, grouped launches in lockstep (qca_group_step)"
+            grp.close()
+        else:
+            res["method"] = "one stream per instance, native worker threads (qca_batch_step)"
+            res["threads"] = a.batch_threads
+            res["gpu_max_hw_queues"] = os.environ.get("GPU_MAX_HW_QUEUES")
+        log(f"batched chi=128 ({method}): {su} site-updates in {dt:.2f} s = {rate:.1f}/s "
+            f"({rate / fp64_rate:.2f}x the plain FP64 single instance)")
+        out[method] = res
+        for e in engines:
+            e.close()
+    out["site_updates_per_s"] = out["grouped"]["site_updates_per_s"]
+    out["speedup_over_fp64_single"] = out["grouped"]["speedup_over_fp64_single"]
+    return out
+
+
 def tdvp1_bench(torch, a):
     """1TDVP at fixed chi = 256: 101 cells, distance 2, interval [2,3], each
     cell alive with p = 0.3 (seed 3), bonds padded to min(2^i, 2^(N-i), 256)
@@ -245,7 +318,8 @@ def serve_bench(a):
     arrivals = serve.poisson_arrivals(a.serve, a.serve_window, seed=5, big_fraction=a.serve_big_fraction,
                                       max_steps=a.serve_max_steps)
     log(f"served window: {len(arrivals)} requests over {a.serve_window:.0f} s")
-    srv = serve.Server(serve.gpu_engine_factory(precision=1, chi_cap=a.serve_chi_cap), workers=a.batch_threads)
+    cap = a.serve_chi_cap or None
+    srv = serve.Server(serve.gpu_engine_factory(precision=1, chi_cap=cap), workers=a.batch_threads)
     out = srv.run(arrivals)
     out.update({"arrival_rate_per_s": a.serve, "window_s": a.serve_window, "max_steps": a.serve_max_steps,
                 "chi_cap": a.serve_chi_cap, "big_fraction": a.serve_big_fraction, "workers": a.batch_threads,
@@ -287,25 +361,28 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mixed-krylov-tol", type=float, default=1e-10,
                     help="Krylov exponential tolerance of the mixed engine (the plain one uses 1e-14); "
                          "its complex64 matvecs are accurate to ~1e-7")
+    ap.add_argument("--batch128", type=int, default=16,
+                    help="instances of the batched chi=128 leg (0: skip)")
+    ap.add_argument("--batch128-warmup", type=int, default=1)
+    ap.add_argument("--batch128-steps", type=int, default=2)
     ap.add_argument("--batch", type=int, default=0,
-                    help="also time the first N instances of the batched 2TDVP workload "
+                    help="also time the first N instances of the batched 2TDVP draws "
                          "(default 0: skip)")
     ap.add_argument("--batch-threads", type=int, default=16)
     ap.add_argument("--batch-warmup", type=int, default=10)
     ap.add_argument("--batch-steps", type=int, default=3)
-    ap.add_argument("--serve", type=float, default=0.0,
-                    help="also run a served window at this Poisson arrival rate (requests/s; "
-                         "default 0: skip)")
-    ap.add_argument("--serve-window", type=float, default=10.0)
-    ap.add_argument("--serve-max-steps", type=int, default=50)
-    ap.add_argument("--serve-chi-cap", type=int, default=32)
-    ap.add_argument("--serve-big-fraction", type=float, default=0.0)
+    ap.add_argument("--serve", type=float, default=50.0,
+                    help="Poisson arrival rate of the served window (requests/s; 0: skip)")
+    ap.add_argument("--serve-window", type=float, default=1.0)
+    ap.add_argument("--serve-max-steps", type=int, default=10)
+    ap.add_argument("--serve-chi-cap", type=int, default=0, help="cap on chi_max of served requests (0: none)")
+    ap.add_argument("--serve-big-fraction", type=float, default=0.1)
     ap.add_argument("--tdvp1", type=int, default=0,
                     help="also time N steps of the 1TDVP chi=256 workload (101 cells), both precisions")
     ap.add_argument("--exact-apply", type=int, default=0,
@@ -324,7 +401,7 @@ def main() -> int:
         return 2
     if a.batch_only and not a.batch:
         a.batch = 32
-    if a.batch or a.serve:
+    if a.batch or a.serve or a.batch128:
         # one hardware queue per worker stream (HIP's default is 4); must be set
         # before the HIP runtime starts
         os.environ["GPU_MAX_HW_QUEUES"] = str(min(16, max(4, a.batch_threads)))
@@ -333,6 +410,13 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("no HIP device visible", file=sys.stderr)
         return 2
+    try:
+        return run(torch, qn, a)
+    finally:
+        qn.shutdown()
+
+
+def run(torch, qn, a) -> int:
     if a.roofline_only:
         ms32, fl = heff2_roofline(torch, 1)
         ms64, _ = heff2_roofline(torch, 0)
@@ -348,23 +432,29 @@ def main() -> int:
                                        "fp64_tflops": fl / (ms64 * 1e-3) / 1e12}}), flush=True)
         return 0
     if a.batch_only:
-        print(json.dumps({"batched": batch_bench(torch, a)}), flush=True)
+        print(json.dumps({"batched_draws": batch_bench(torch, a)}), flush=True)
         return 0
     Ws = mpo().W
     psi = initial_state()
-    # W untimed steps of the plain engine grow the bonds towards chi_max; the
+    # Preparation: PREP_STEPS plain FP64 steps grow the bonds to chi_max; the
     # mixed engine then starts from that same state.
     engines = {"fp64": qn.Engine(Ws, psi.A, algorithm=2, precision=0, chi_max=CHI, dt=DT, eps=EPS,
                                  krylov_tol=1e-14)}
-    log(f"engine built; warmup {a.warmup} steps")
-    for w in range(a.warmup):
+    log(f"engine built; preparation {PREP_STEPS} FP64 steps")
+    for w in range(PREP_STEPS):
         engines["fp64"].step(1)
-        if w % 5 == 4 or w == a.warmup - 1:
+        if w % 5 == 4 or w == PREP_STEPS - 1:
             bd = engines["fp64"].bond_dims()
-            log(f"warmup {w + 1}: max bond {bd.max()}, bonds at chi_max {int((bd == CHI).sum())}")
-    engines = {"mixed": qn.Engine(Ws, engines["fp64"].tensors(), algorithm=2, precision=1, chi_max=CHI,
+            log(f"prep {w + 1}: max bond {bd.max()}, bonds at chi_max {int((bd == CHI).sum())}")
+    prepared = engines["fp64"].tensors()
+    engines = {"mixed": qn.Engine(Ws, prepared, algorithm=2, precision=1, chi_max=CHI,
                                   dt=DT, eps=EPS, krylov_tol=a.mixed_krylov_tol),
                "fp64": engines["fp64"]}
+    for k in range(a.warmup):
+        for name, e in engines.items():
+            e.step(1)
+        log(f"warmup {k + 1}/{a.warmup}")
+    bonds_start = {k: e.bond_dims() for k, e in engines.items()}
     maxdiff = 0.0
     times = {"mixed": 0.0, "fp64": 0.0}
     gpu_ms = {"mixed": 0.0, "fp64": 0.0}
@@ -386,12 +476,16 @@ def main() -> int:
         flops[name] = e.stats()
     su = a.steps * 2 * (N - 1)
     rate = {k: su / v for k, v in times.items()}
-    bonds = engines["fp64"].bond_dims()
+    bonds_end = {k: e.bond_dims() for k, e in engines.items()}
     log("roofline micro-benchmark of the 2-site H_eff application")
     ms32, fl = heff2_roofline(torch, 1)
     ms64, _ = heff2_roofline(torch, 0)
     ach32 = fl / (ms32 * 1e-3) / 1e12
     ach64 = fl / (ms64 * 1e-3) / 1e12
+    fp64_tensors = engines["fp64"].tensors()
+    for e in engines.values():
+        e.close()
+    batched128 = batch128_bench(torch, a, prepared, rate["fp64"]) if a.batch128 else None
     tdvp1 = tdvp1_bench(torch, a) if a.tdvp1 else None
     exact = exact_apply_bench(torch, a.exact_apply) if a.exact_apply else None
     batched = batch_bench(torch, a) if a.batch else None
@@ -399,7 +493,7 @@ def main() -> int:
     cpu = None
     if not a.no_cpu_baseline:
         log(f"cpu baseline ({a.cpu_seconds:.0f} s budget)")
-        cpu = cpu_baseline(engines["fp64"].tensors(), a.cpu_seconds)
+        cpu = cpu_baseline(fp64_tensors, a.cpu_seconds)
     out = {
         "metric": "2TDVP site-updates/s at \u03c7=128, mixed vs FP64 @max|\u0394\u27e8n\u27e9|\u22641e-6; "
                   "p50/p99 latency",
@@ -413,12 +507,15 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c64 contractions + f64 Krylov/SVD",
-        "data": "synthetic (blinker initial state, rule Hamiltonian MPO)",
+        "data": "synthetic (three adjacent centre cells alive, rule Hamiltonian MPO)",
         "config": {"workload": "2TDVP single instance: 41 cells, distance 2, interval [2,4], three centre "
-                               "cells alive, dt 0.1, chi_max 128, eps 1e-10",
+                               "cells alive, dt 0.1, chi_max 128, eps 1e-10; timed on the state after "
+                               f"{PREP_STEPS} preparation FP64 steps plus the warmup steps",
                    "ncells": N, "chi_max": CHI, "mpo_bond": int(Ws[N // 2].shape[2]),
-                   "bond_dims_after_warmup_max": int(bonds.max()),
-                   "bonds_at_chi_max": int((bonds == CHI).sum())},
+                   "prep_steps": PREP_STEPS,
+                   "bonds_at_chi_max": {k: {"start": int((bonds_start[k] == CHI).sum()),
+                                            "end": int((bonds_end[k] == CHI).sum())} for k in bonds_start},
+                   "bonds_total": N - 1},
         "fp64_value": rate["fp64"],
         "mixed_over_fp64": rate["mixed"] / rate["fp64"],
         "max_abs_dn_mixed_vs_fp64": maxdiff,
@@ -431,16 +528,18 @@ def main() -> int:
         "roofline": {"bound": "mfma", "kernel": "2-site H_eff application (mixed, complex64 on FP32 MFMA)",
                      "achieved": ach32, "peak": 157.3, "unit": "TFLOP/s", "frac": ach32 / 157.3,
                      "traffic": HEFF2_TRAFFIC_BYTES["mixed"], "traffic_unit": "B per launch",
-                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --roofline-only, "
-                                       "scaled by a 256 MiB copy (profiles/heff2_chi128_d13_hbm_bytes.md)",
+                     "traffic_source": "profile-derived (not measured in this run): rocprofv3 --pmc "
+                                       "FETCH_SIZE / WRITE_SIZE of bench.py --roofline-only, scaled by a "
+                                       f"256 MiB copy ({HEFF2_TRAFFIC_SOURCE})",
                      "ms_per_launch": ms32, "flops_per_launch": fl,
                      "fp64": {"achieved": ach64, "peak": 78.6, "frac": ach64 / 78.6, "ms_per_launch": ms64,
                               "traffic": HEFF2_TRAFFIC_BYTES["fp64"]}},
         "cpu_baseline": cpu,
+        "batched_chi128": batched128,
+        "served": served,
         "tdvp1_chi256": tdvp1,
         "exact_apply": exact,
-        "batched": batched,
-        "served": served,
+        "batched_draws": batched,
     }
     print(json.dumps(out), flush=True)
     return 0

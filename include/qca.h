@@ -10,9 +10,13 @@
  *
  * Conventions
  *  - Every function returns 0 on success and a negative code on failure
- *    (-1 internal, -2 invalid argument, -3 HIP runtime error); qca_last_error()
- *    returns the message of the last failure on the calling thread.  No C++
- *    exception crosses this boundary.
+ *    (-1 internal, -2 invalid argument, -3 HIP runtime error, -4 an iterative
+ *    method did not converge: a Jacobi SVD that made rotations in every one
+ *    of its 60 sweeps, or a Krylov exponential fed non-finite numbers);
+ *    qca_last_error() returns the message of the last failure on the calling
+ *    thread.  No C++ exception crosses this boundary.  An engine whose time
+ *    step failed part-way refuses every later call except destroy (its
+ *    tensors and cached environments no longer agree).
  *  - Complex numbers are interleaved (re, im) doubles (complex128) or floats
  *    (complex64), the layout of numpy/torch complex arrays.
  *  - Tensor layouts are the reference's: MPS site A[p][a][b] (2, chi_l, chi_r),
@@ -37,6 +41,8 @@
 #ifndef QCA_H
 #define QCA_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -45,6 +51,7 @@ extern "C" {
 #define QCA_ERR_INTERNAL (-1)
 #define QCA_ERR_ARG (-2)
 #define QCA_ERR_HIP (-3)
+#define QCA_ERR_NOT_CONVERGED (-4)
 
 /* Library version (major*10000 + minor*100 + patch). */
 int qca_version(void);
@@ -91,6 +98,19 @@ typedef struct qca_stats {
  * tdvp.py:24-26) and builds all right environments (tdvp.py:37-39). */
 int qca_engine_create(const qca_engine_params* params, const double* const* W, const int* Wdl,
                       const int* Wdr, const double* const* A, const int* chis, qca_engine** out);
+/* Device bytes qca_engine_create with the same arguments (the state itself
+ * is not needed) would allocate: MPS and environments at their bond
+ * capacities, sparse MPO forms, Krylov basis and every workspace.  Computed on
+ * the host by the constructor's own sizing; needs no device.  No counterpart
+ * in the reference (admission of served requests by device footprint). */
+int qca_engine_footprint(const qca_engine_params* params, const double* const* W, const int* Wdl,
+                         const int* Wdr, const int* chis, size_t* bytes);
+/* Device bytes held by an engine (the same sum, from its allocations).  No
+ * counterpart in the reference. */
+int qca_engine_device_bytes(qca_engine* e, size_t* bytes);
+/* Free and total device memory of a HIP device (hipMemGetInfo).  No
+ * counterpart in the reference. */
+int qca_device_memory(int device, size_t* free_bytes, size_t* total_bytes);
 /* Frees the engine's device state (the reference's TDVP object going out of
  * scope, algorithms/tdvp.py:10). */
 int qca_engine_destroy(qca_engine* e);
@@ -142,6 +162,44 @@ int qca_engine_set_fused_krylov_limit(qca_engine* e, long max_n);
  * (quantum_game.py:75-119 loops over initial states sequentially). */
 int qca_batch_step(qca_engine* const* engines, int n, int nsteps, int nthreads, int* failed);
 
+/* ------------------------------------------------------------------ groups
+ * Lockstep 2TDVP of many engines with grouped launches: at every sweep
+ * position each kernel runs once for all instances (one record per instance
+ * holding its operands and shapes), with one host round trip per position
+ * for the truncation ranks of all splits and one per chunk of Krylov
+ * iterations for all convergence flags.  Every instance computes exactly what
+ * qca_engine_step computes for it alone (same GEMM plans, kernels,
+ * reductions and stopping tests), so results are bit-identical.  Instances
+ * may differ in cell count, bond cap, MPO, time step and cutoff.  No
+ * counterpart in the reference, which runs one simulation per process
+ * (quantum_game.py:75-119); each instance's step is reference
+ * TDVP.do_time_step (tdvp.py:50-63, 107-145).
+ */
+typedef struct qca_group qca_group;
+
+typedef struct qca_group_stats {
+  long steps;              /* group time steps taken */
+  long krylov_iterations;  /* grouped Lanczos iterations launched */
+  long host_syncs;         /* batch-wide host round trips */
+  double krylov_ms;        /* device time of the multi-launch Krylov phases */
+  double svd_ms;           /* device time of the splits */
+  double total_ms;         /* device time of the group steps */
+} qca_group_stats;
+
+/* Groups n engines: all 2TDVP, one device, one precision, each at most once.
+ * The engines stay owned by the caller and must outlive the group; they can
+ * still be measured, read and stepped alone between group steps (not
+ * concurrently with them). */
+int qca_group_create(qca_engine* const* engines, int n, qca_group** out);
+/* nsteps time steps of every engine of the group.  A failure marks every
+ * engine of the group as failed (their sweeps stopped part-way). */
+int qca_group_step(qca_group* g, int nsteps);
+/* Counters since creation or the last reset. */
+int qca_group_get_stats(qca_group* g, qca_group_stats* out);
+int qca_group_reset_stats(qca_group* g);
+/* Frees the group's records and buffers (not the engines). */
+int qca_group_destroy(qca_group* g);
+
 /* ------------------------------------------------------------------ primitives */
 
 /* y = H_eff x (matrix-free).  Replaces the dense assembly + product of
@@ -190,7 +248,11 @@ int qca_exact_apply_h(int precision, const void* psi, void* out, int ncells, int
                       int hi, void* stream);
 
 /* psi <- exp(-i (pi/2) dt H) psi on the 2^N basis (complex128, device) by
- * Krylov with the closed-form H.  Replaces reference exact.py:15-28. */
+ * Krylov with the closed-form H.  Replaces reference exact.py:15-28.  The
+ * step is split into 2^l equal substeps when the Krylov basis (at most 64
+ * vectors, at least 24, within 16 GiB) cannot hold the whole step: a substep
+ * that reaches the basis limit is discarded and retried at half the length.
+ * *iters receives the matvec count summed over the substeps. */
 int qca_exact_expm(void* psi, int ncells, int distance, int lo, int hi, double dt, double tol,
                    int* iters, void* stream);
 

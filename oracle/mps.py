@@ -115,9 +115,10 @@ def measure_reference(As: list):
     return pop, np.round(pop), sse, bd
 
 
-def measure_consistent(As: list):
-    """Same observables as reference tensor_networks/mps.py:100-140 without
-    moving the orthogonality centre.
+def rho_consistent(As: list) -> np.ndarray:
+    """Single-site density matrices (N, 2, 2) without moving the orthogonality
+    centre: the rho the reference reads at each centre site
+    (tensor_networks/mps.py:118-132).
 
     Requires the TDVP invariant that sites 1..N-1 are right-isometric (centre
     at site 0): then rho_i = sum E_{i-1}[a,a'] A_i[p,a,b] conj(A_i[p',a',b])
@@ -125,16 +126,24 @@ def measure_consistent(As: list):
     This equals the reference's centre-site rho for every site.
     """
     n = len(As)
-    bd = bond_dims(As)
-    pop = np.zeros(n)
-    sse = np.zeros(n)
+    out = np.zeros((n, 2, 2), dtype=complex)
     E = np.ones((1, 1), dtype=complex)
     for site in range(n):
         A = As[site]
-        rho = np.einsum("xy,pxb,qyb->pq", E, A, A.conj())
-        pop[site] = rho[1, 1].real
-        sse[site] = _entropy_bits(rho)
-        E = np.einsum("xy,pxb,pyc->bc", E, A, A.conj())
+        EA = np.einsum("xy,pxb->pyb", E, A, optimize=True)
+        out[site] = np.einsum("pyb,qyb->pq", EA, A.conj(), optimize=True)
+        E = np.einsum("pyb,pyc->bc", EA, A.conj(), optimize=True)
+    return out
+
+
+def measure_consistent(As: list):
+    """Same observables as reference tensor_networks/mps.py:100-140 without
+    moving the orthogonality centre (densities from rho_consistent)."""
+    n = len(As)
+    bd = bond_dims(As)
+    rho = rho_consistent(As)
+    pop = np.array([rho[i, 1, 1].real for i in range(n)])
+    sse = np.array([_entropy_bits(rho[i]) for i in range(n)])
     return pop, np.round(pop), sse, bd
 
 

@@ -21,7 +21,8 @@ BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libqca.so")
 
-SOURCES = ["gemm.hip", "kernels.hip", "krylov_fused.hip", "svd.hip", "contract.cpp", "engine.cpp", "capi.cpp"]
+SOURCES = ["gemm.hip", "kernels.hip", "krylov_fused.hip", "svd.hip", "contract.cpp", "engine.cpp", "group.cpp",
+           "capi.cpp"]
 ARCH = "gfx950"
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "group.hpp"
 
 namespace {
 
@@ -123,29 +124,45 @@ qca::LanczosWork lanczos_ws(long n, qca::LanczosCtrl** ctrl_host, int max_m = qc
   lw.part_norm = sbuf<double>("lz_pnorm", lw.max_blocks);
   lw.coef1 = sbuf<qca::c128>("lz_c1", qca::LZ_MAXK);
   lw.coef2 = sbuf<qca::c128>("lz_c2", qca::LZ_MAXK);
-  static thread_local qca::LanczosCtrl* host = nullptr;
-  if (!host) QCA_HIP(hipHostMalloc((void**)&host, sizeof(qca::LanczosCtrl)));
-  *ctrl_host = host;
+  // ordinary host memory: the control block is read once per exponential
+  // after a stream synchronisation, and no HIP resource outlives the calls
+  static thread_local qca::LanczosCtrl host;
+  *ctrl_host = &host;
   return lw;
 }
 
-// Runs the Krylov loop: mv(j, V_j) must enqueue w = H V_j and the iteration tail.
+struct KrylovOutcome {
+  int m = 0;           // Krylov dimension used
+  bool hit_limit = false;
+};
+
+// Runs the Krylov loop: mv(j, V_j) must enqueue w = H V_j and the iteration
+// tail.  Iterations j = 0 .. max_m-2 may run; the device closes the
+// exponential at the latest at dimension max_m-1 and then flags a limit hit
+// when its error estimate is still above the tolerance.  x is overwritten with
+// the result unless the limit was hit and keep_on_limit is false.
 template <typename MV>
-int krylov_loop(qca::c128* x, long n, qca::LanczosWork& lw, qca::LanczosCtrl* host, MV&& mv,
-                hipStream_t s) {
+KrylovOutcome krylov_loop(qca::c128* x, long n, qca::LanczosWork& lw, qca::LanczosCtrl* host,
+                          MV&& mv, hipStream_t s, bool keep_on_limit = true) {
   qca::lanczos_begin(x, n, lw, s);
   int j = 0, chunk = 8;
   while (true) {
-    for (int c = 0; c < chunk && j < lw.max_m - 2; ++c, ++j) mv(j, lw.V + (long)j * n);
+    for (int c = 0; c < chunk && j < lw.max_m - 1; ++c, ++j) mv(j, lw.V + (long)j * n);
     QCA_HIP(hipMemcpyAsync(host, lw.ctrl, sizeof(qca::LanczosCtrl), hipMemcpyDeviceToHost, s));
     QCA_HIP(hipStreamSynchronize(s));
     if (host->done) break;
-    QCA_CHECK(j < lw.max_m - 2, "Krylov exponential did not converge");
+    if (j >= lw.max_m - 1)
+      throw qca::Error(qca::QCA_ERR_NOT_CONVERGED_CODE,
+                       "Krylov exponential did not converge (non-finite input?)");
     chunk = 4;
   }
+  KrylovOutcome r;
+  r.m = host->m;
+  r.hit_limit = host->hit_limit != 0;
+  if (r.hit_limit && !keep_on_limit) return r;
   qca::lanczos_finish(n, x, lw, s);
   QCA_HIP(hipStreamSynchronize(s));
-  return host->m;
+  return r;
 }
 
 }  // namespace
@@ -154,16 +171,32 @@ struct qca_engine {
   qca::Engine* eng = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double gpu_ms = 0;
+  // Set when a time step fails part-way: the tensors and environments are then
+  // mutually inconsistent, so every later call on the engine is refused.
+  std::string failed;
 };
 
 namespace {
 
+void require_usable(const qca_engine* e) {
+  QCA_CHECK(e && e->eng, "null engine");
+  if (!e->failed.empty())
+    throw qca::Error(QCA_ERR_ARG, "engine unusable after an earlier failed step: " + e->failed);
+}
+
 // One time step of `e` on its stream, with its GPU time added to e->gpu_ms.
+// A failure marks the engine as failed.
 void engine_one_step(qca_engine* e) {
-  QCA_HIP(hipEventRecord(e->ev0, e->eng->stream()));
-  e->eng->step();
-  QCA_HIP(hipEventRecord(e->ev1, e->eng->stream()));
-  QCA_HIP(hipEventSynchronize(e->ev1));
+  require_usable(e);
+  try {
+    QCA_HIP(hipEventRecord(e->ev0, e->eng->stream()));
+    e->eng->step();
+    QCA_HIP(hipEventRecord(e->ev1, e->eng->stream()));
+    QCA_HIP(hipEventSynchronize(e->ev1));
+  } catch (const std::exception& ex) {
+    e->failed = ex.what();
+    throw;
+  }
   float ms = 0;
   QCA_HIP(hipEventElapsedTime(&ms, e->ev0, e->ev1));
   e->gpu_ms += ms;
@@ -184,20 +217,63 @@ int qca_device_count(int* count) {
   });
 }
 
+namespace {
+qca::EngineParams to_params(const qca_engine_params* params) {
+  const int n = params->ncells;
+  QCA_CHECK(n >= 2 && n <= 4096, "ncells out of range");
+  qca::EngineParams p;
+  p.ncells = n;
+  p.algorithm = params->algorithm;
+  p.precision = params->precision;
+  p.chi_max = params->chi_max;
+  p.dt = params->dt;
+  p.eps = params->eps;
+  p.krylov_tol = params->krylov_tol > 0 ? params->krylov_tol : 1e-14;
+  return p;
+}
+}  // namespace
+
+int qca_engine_footprint(const qca_engine_params* params, const double* const* W, const int* Wdl,
+                         const int* Wdr, const int* chis, size_t* bytes) {
+  return guarded([&] {
+    QCA_CHECK(params && W && Wdl && Wdr && chis && bytes, "null argument");
+    const qca::EngineParams p = to_params(params);
+    const int n = p.ncells;
+    std::vector<const qca::c128*> Wv(n);
+    for (int i = 0; i < n; ++i) {
+      QCA_CHECK(W[i], "null tensor pointer");
+      Wv[i] = reinterpret_cast<const qca::c128*>(W[i]);
+    }
+    std::vector<int> dl(Wdl, Wdl + n), dr(Wdr, Wdr + n), ch(chis, chis + n + 1);
+    *bytes = qca::Engine::footprint(p, Wv, dl, dr, ch);
+  });
+}
+
+int qca_engine_device_bytes(qca_engine* e, size_t* bytes) {
+  return guarded([&] {
+    QCA_CHECK(e && e->eng && bytes, "null argument");
+    *bytes = e->eng->device_bytes();
+  });
+}
+
+int qca_device_memory(int device, size_t* free_bytes, size_t* total_bytes) {
+  return guarded([&] {
+    QCA_CHECK(free_bytes && total_bytes, "null argument");
+    int cur = 0;
+    QCA_HIP(hipGetDevice(&cur));
+    QCA_HIP(hipSetDevice(device));
+    const hipError_t rc = hipMemGetInfo(free_bytes, total_bytes);
+    (void)hipSetDevice(cur);
+    QCA_HIP(rc);
+  });
+}
+
 int qca_engine_create(const qca_engine_params* params, const double* const* W, const int* Wdl,
                       const int* Wdr, const double* const* A, const int* chis, qca_engine** out) {
   return guarded([&] {
     QCA_CHECK(params && W && Wdl && Wdr && A && chis && out, "null argument");
-    const int n = params->ncells;
-    QCA_CHECK(n >= 2 && n <= 4096, "ncells out of range");
-    qca::EngineParams p;
-    p.ncells = n;
-    p.algorithm = params->algorithm;
-    p.precision = params->precision;
-    p.chi_max = params->chi_max;
-    p.dt = params->dt;
-    p.eps = params->eps;
-    p.krylov_tol = params->krylov_tol > 0 ? params->krylov_tol : 1e-14;
+    const qca::EngineParams p = to_params(params);
+    const int n = p.ncells;
     std::vector<const qca::c128*> Wv(n), Av(n);
     for (int i = 0; i < n; ++i) {
       QCA_CHECK(W[i] && A[i], "null tensor pointer");
@@ -293,9 +369,79 @@ int qca_batch_step(qca_engine* const* engines, int n, int nsteps, int nthreads, 
   return QCA_OK;
 }
 
+struct qca_group {
+  qca::Group* grp = nullptr;
+  std::vector<qca_engine*> engines;
+};
+
+int qca_group_create(qca_engine* const* engines, int n, qca_group** out) {
+  return guarded([&] {
+    QCA_CHECK(engines && out && n >= 1, "null argument or empty group");
+    QCA_CHECK(n <= 4096, "at most 4096 engines per group");
+    std::vector<qca::Engine*> v;
+    for (int i = 0; i < n; ++i) {
+      require_usable(engines[i]);
+      v.push_back(engines[i]->eng);
+    }
+    auto* g = new qca_group();
+    try {
+      g->grp = new qca::Group(v);
+    } catch (...) {
+      delete g;
+      throw;
+    }
+    g->engines.assign(engines, engines + n);
+    *out = g;
+  });
+}
+
+int qca_group_step(qca_group* g, int nsteps) {
+  return guarded([&] {
+    QCA_CHECK(g && g->grp, "null group");
+    QCA_CHECK(nsteps >= 0, "negative step count");
+    for (qca_engine* e : g->engines) require_usable(e);
+    try {
+      g->grp->step(nsteps);
+    } catch (const std::exception& ex) {
+      // the instances stopped part-way through a sweep
+      for (qca_engine* e : g->engines) e->failed = ex.what();
+      throw;
+    }
+  });
+}
+
+int qca_group_get_stats(qca_group* g, qca_group_stats* out) {
+  return guarded([&] {
+    QCA_CHECK(g && g->grp && out, "null argument");
+    const qca::GroupStats& s = g->grp->stats();
+    out->steps = s.steps;
+    out->krylov_iterations = s.launches_krylov;
+    out->host_syncs = s.host_syncs;
+    out->krylov_ms = s.krylov_ms;
+    out->svd_ms = s.svd_ms;
+    out->total_ms = s.total_ms;
+  });
+}
+
+int qca_group_reset_stats(qca_group* g) {
+  return guarded([&] {
+    QCA_CHECK(g && g->grp, "null group");
+    g->grp->reset_stats();
+  });
+}
+
+int qca_group_destroy(qca_group* g) {
+  return guarded([&] {
+    if (!g) return;
+    delete g->grp;
+    delete g;
+  });
+}
+
 int qca_engine_measure(qca_engine* e, double* rho_out) {
   return guarded([&] {
     QCA_CHECK(e && e->eng && rho_out, "null argument");
+    require_usable(e);
     e->eng->measure(reinterpret_cast<qca::c128*>(rho_out));
   });
 }
@@ -305,6 +451,7 @@ int qca_engine_observables(qca_engine* e, double* population, double* d_populati
   return guarded([&] {
     QCA_CHECK(e && e->eng && population && d_population && single_site_entropy && bond_dims,
               "null argument");
+    require_usable(e);
     QCA_HIP(hipSetDevice(e->eng->device()));
     const int n = e->eng->ncells();
     for (int i = 0; i <= n; ++i) bond_dims[i] = e->eng->chi(i);
@@ -322,6 +469,7 @@ int qca_engine_bond_dims(qca_engine* e, int* out) {
 int qca_engine_get_site(qca_engine* e, int site, double* out) {
   return guarded([&] {
     QCA_CHECK(e && e->eng && out, "null argument");
+    require_usable(e);
     e->eng->get_site(site, reinterpret_cast<qca::c128*>(out));
   });
 }
@@ -451,7 +599,7 @@ int qca_krylov_expm(int precision, int kind, void* x, const void* L, const void*
                                   wp ? &wp->d64 : nullptr, Din, Dout, chil, chir, ws,
                                   qca::lanczos_done_flag(lw), s);
           qca::lanczos_iteration_tail<double>(y, n, j, tau, tl, lw, s);
-        }, s);
+        }, s).m;
       } else {
         auto ws = contract_ws<float>(tsize, n);
         qca::c64* x32 = sbuf<qca::c64>("kr_x32", n);
@@ -462,7 +610,7 @@ int qca_krylov_expm(int precision, int kind, void* x, const void* L, const void*
                                  wp ? &wp->d32 : nullptr, Din, Dout, chil, chir, ws,
                                  qca::lanczos_done_flag(lw), s);
           qca::lanczos_iteration_tail<float>(y, n, j, tau, tl, lw, s);
-        }, s);
+        }, s).m;
       }
       if (iters) *iters = m;
     }
@@ -485,17 +633,22 @@ int qca_svd_split(const void* theta, void* Al, void* Ar, int chil, int chir, int
     int* perm = sbuf<int>("svd_perm", cols);
     qca::SvdResult* res = sbuf<qca::SvdResult>("svd_res", 1);
     qca::launch_theta_to_svd(th, X, chil, chir, right_sweep ? 0 : 1, s);
-    static thread_local qca::JacobiWork jw;  // keeps the pinned word pair and the sweep count
+    // keeps the sweep count of the previous call; the convergence words are
+    // read into ordinary host memory (no HIP resource outlives the call)
+    static thread_local qca::JacobiWork jw;
+    static thread_local long long jw_words[8];
     jw.ws_bytes = qca::jacobi_workspace_bytes(rows, cols);
     jw.ws = sbuf<unsigned char>("svd_ws", (long)jw.ws_bytes);
     jw.pre_elems = qca::jacobi_precond_elems(rows, cols);
     jw.pre = sbuf<qca::c128>("svd_pre", (long)jw.pre_elems);
-    if (!jw.host) QCA_HIP(hipHostMalloc(&jw.host, 64));
+    jw.host = jw_words;
     qca::launch_jacobi_svd(X, rows, cols, sigma, perm, res, chi_max, eps, rank_cap, s, jw);
     qca::SvdResult hres;
     QCA_HIP(hipMemcpyAsync(&hres, res, sizeof(hres), hipMemcpyDeviceToHost, s));
     QCA_HIP(hipStreamSynchronize(s));
     const int kk = hres.k;
+    if (!hres.converged)
+      throw qca::Error(qca::QCA_ERR_NOT_CONVERGED_CODE, "Jacobi SVD did not converge");
     QCA_CHECK(kk >= 1, "empty truncation");
     qca::ContractWs<double> ws = contract_ws<double>(4L * chil * chir, 4L * chil * chir);
     const double inv = 1.0 / hres.norm_kept;
@@ -573,6 +726,7 @@ int qca_exact_expm(void* psi, int ncells, int distance, int lo, int hi, double d
     QCA_CHECK(psi, "null vector");
     QCA_CHECK(ncells >= 2 && ncells <= 28, "ncells out of range for the Krylov exponential (2..28)");
     QCA_CHECK(distance >= 1, "distance must be positive");
+    QCA_CHECK(std::isfinite(dt), "time step must be finite");
     hipStream_t s = as_stream(stream);
     const long n = 1L << ncells;
     // Krylov basis capped at 16 GiB (at least 24 vectors)
@@ -580,13 +734,37 @@ int qca_exact_expm(void* psi, int ncells, int distance, int lo, int hi, double d
     qca::LanczosCtrl* host = nullptr;
     qca::LanczosWork lw = lanczos_ws(n, &host, max_m);
     qca::c128* y = sbuf<qca::c128>("ex_y", n);
-    const double tau = 1.5707963267948966 * dt;
     const double tl = tol > 0 ? tol : 1e-14;
-    const int m = krylov_loop((qca::c128*)psi, n, lw, host, [&](int j, const qca::c128* vj) {
-      qca::launch_exact_apply<double>(vj, y, ncells, distance, lo, hi, s);
-      qca::lanczos_iteration_tail<double>(y, n, j, tau, tl, lw, s);
-    }, s);
-    if (iters) *iters = m;
+    // The time step is split into 2^level equal substeps.  A short basis
+    // starts from about one substep per 8 units of |tau| * N (each row of H
+    // has at most N unit entries, so ||H|| <= N); a substep whose exponential
+    // reaches the basis limit leaves psi untouched and is retried at half the
+    // length.  Progress is counted in units of dt / 2^level, so every substep
+    // is an exact binary fraction of dt.
+    const double tau_full = 1.5707963267948966 * dt;
+    int level = 0;
+    if (max_m < 48) {
+      const double est = std::fabs(tau_full) * ncells / 8.0;
+      while (level < 20 && (double)(1L << level) < est) ++level;
+    }
+    long done_units = 0;
+    int total = 0;
+    while (done_units < (1L << level)) {
+      const double tau = tau_full / (double)(1L << level);
+      const KrylovOutcome r = krylov_loop((qca::c128*)psi, n, lw, host, [&](int j, const qca::c128* vj) {
+        qca::launch_exact_apply<double>(vj, y, ncells, distance, lo, hi, s);
+        qca::lanczos_iteration_tail<double>(y, n, j, tau, tl, lw, s);
+      }, s, /*keep_on_limit=*/false);
+      total += r.m;
+      if (r.hit_limit) {
+        QCA_CHECK(level < 30, "Krylov exponential of the state vector did not converge");
+        ++level;
+        done_units *= 2;
+        continue;
+      }
+      ++done_units;
+    }
+    if (iters) *iters = total;
   });
 }
 

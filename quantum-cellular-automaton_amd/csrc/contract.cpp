@@ -30,9 +30,10 @@ void gemm_t<float>(const GemmDesc& d, const c64* A, const c64* B, c64* C, hipStr
 }
 
 template <typename T>
-void heff_apply(int kind, const cplx<T>* x, cplx<T>* y, const cplx<T>* L, const cplx<T>* R,
-                const WSparseDev<T>* W, int Din, int Dout, int chil, int chir,
-                const ContractWs<T>& ws, const int* skip, hipStream_t s) {
+ContractPlan<T> heff_plan(int kind, const cplx<T>* x, cplx<T>* y, const cplx<T>* L, const cplx<T>* R,
+                          const WSparseDev<T>* W, int Din, int Dout, int chil, int chir,
+                          const ContractWs<T>& ws, const int* skip) {
+  ContractPlan<T> p{};
   const long lr = (long)chil * chir;
   if (kind == 2) {
     const int D = Din;
@@ -42,7 +43,7 @@ void heff_apply(int kind, const cplx<T>* x, cplx<T>* y, const cplx<T>* L, const 
     g.b_k2 = chir; g.b_n = 1;
     g.c_m = chir; g.c_n = 1;
     g.skip = skip;
-    gemm_t<T>(g, L, x, ws.T1, s, ws);
+    p.g1 = GemmJob<T>{g, L, x, ws.T1, ws.splitk, ws.splitk_elems};
     // y[x][y] = sum_{(w,c)} T[w][x][c] R[c][w][y]
     GemmDesc h = gemm_desc(chil, chir, D * chir);
     h.K2 = chir;
@@ -50,8 +51,9 @@ void heff_apply(int kind, const cplx<T>* x, cplx<T>* y, const cplx<T>* L, const 
     h.b_k1 = chir; h.b_k2 = (long)D * chir; h.b_n = 1;
     h.c_m = chir; h.c_n = 1;
     h.skip = skip;
-    gemm_t<T>(h, ws.T1, R, y, s, ws);
-    return;
+    p.g3 = GemmJob<T>{h, ws.T1, R, y, ws.splitk, ws.splitk_elems};
+    p.mix = false;
+    return p;
   }
   const int dq = kind == 0 ? 4 : 2;
   // GEMM1: T[q][(w,x)][c] = sum_a L[a][(w,x)] x[q][a][c]
@@ -61,9 +63,10 @@ void heff_apply(int kind, const cplx<T>* x, cplx<T>* y, const cplx<T>* L, const 
   g.b_k2 = chir; g.b_n = 1; g.b_b2 = lr;
   g.c_m = chir; g.c_n = 1; g.c_b2 = (long)Din * lr;
   g.skip = skip;
-  gemm_t<T>(g, L, x, ws.T1, s, ws);
+  p.g1 = GemmJob<T>{g, L, x, ws.T1, ws.splitk, ws.splitk_elems};
   // U[q'][(x,c)][v] = sum_{q,w} W[q][q'][w][v] T[q][w][(x,c)]
-  launch_wmix<T>(ws.T1, lr, lr, dq * Din, ws.U, dq, Dout, *W, s, skip);
+  p.mix = true;
+  p.wm = WmixArgs<T>{ws.T1, lr, lr, dq * Din, ws.U, dq, Dout, W->rowptr, W->col, W->val, skip};
   // GEMM3: y[q'][x][y] = sum_{(c,v)} U[q'][x][(c,v)] R[(c,v)][y]
   GemmDesc h = gemm_desc(chil, chir, chir * Dout);
   h.NB2 = dq;
@@ -71,14 +74,17 @@ void heff_apply(int kind, const cplx<T>* x, cplx<T>* y, const cplx<T>* L, const 
   h.b_k2 = chir; h.b_n = 1;
   h.c_m = chir; h.c_n = 1; h.c_b2 = lr;
   h.skip = skip;
-  gemm_t<T>(h, ws.U, R, y, s, ws);
+  p.g3 = GemmJob<T>{h, ws.U, R, y, ws.splitk, ws.splitk_elems};
+  return p;
 }
 
 template <typename T>
-void env_update(bool left, const cplx<T>* A, const cplx<T>* Eprev, cplx<T>* Enew,
-                const WSparseDev<T>* W, int Din, int Dout, int chil, int chir,
-                const ContractWs<T>& ws, hipStream_t s) {
+ContractPlan<T> env_plan(bool left, const cplx<T>* A, const cplx<T>* Eprev, cplx<T>* Enew,
+                         const WSparseDev<T>* W, int Din, int Dout, int chil, int chir,
+                         const ContractWs<T>& ws) {
+  ContractPlan<T> p{};
   const long lr = (long)chil * chir;
+  p.mix = true;
   if (left) {
     // T[p][(w,x)][b] = sum_a L[a][(w,x)] A[p][a][b]
     GemmDesc g = gemm_desc(Din * chil, chir, chil);
@@ -86,14 +92,14 @@ void env_update(bool left, const cplx<T>* A, const cplx<T>* Eprev, cplx<T>* Enew
     g.a_m = 1; g.a_k2 = (long)Din * chil;
     g.b_k2 = chir; g.b_n = 1; g.b_b2 = lr;
     g.c_m = chir; g.c_n = 1; g.c_b2 = (long)Din * lr;
-    gemm_t<T>(g, Eprev, A, ws.T1, s, ws);
-    launch_wmix<T>(ws.T1, lr, lr, 2 * Din, ws.U, 2, Dout, *W, s);
+    p.g1 = GemmJob<T>{g, Eprev, A, ws.T1, ws.splitk, ws.splitk_elems};
+    p.wm = WmixArgs<T>{ws.T1, lr, lr, 2 * Din, ws.U, 2, Dout, W->rowptr, W->col, W->val, nullptr};
     // Lnew[(b,w')][b*] = sum_{(p',x)} U[(p',x)][(b,w')] conj(A[(p',x)][b*])
     GemmDesc h = gemm_desc(chir * Dout, chir, 2 * chil);
     h.a_m = 1; h.a_k2 = (long)chir * Dout;
     h.b_k2 = chir; h.b_n = 1; h.conj_b = 1;
     h.c_m = chir; h.c_n = 1;
-    gemm_t<T>(h, ws.U, A, Enew, s, ws);
+    p.g3 = GemmJob<T>{h, ws.U, A, Enew, ws.splitk, ws.splitk_elems};
   } else {
     // T[p][(w',y)][a] = sum_b R[b][(w',y)] A[p][a][b]
     GemmDesc g = gemm_desc(Din * chir, chil, chir);
@@ -101,18 +107,57 @@ void env_update(bool left, const cplx<T>* A, const cplx<T>* Eprev, cplx<T>* Enew
     g.a_m = 1; g.a_k2 = (long)Din * chir;
     g.b_k2 = 1; g.b_n = chir; g.b_b2 = lr;
     g.c_m = chil; g.c_n = 1; g.c_b2 = (long)Din * lr;
-    gemm_t<T>(g, Eprev, A, ws.T1, s, ws);
-    launch_wmix<T>(ws.T1, lr, lr, 2 * Din, ws.U, 2, Dout, *W, s);
+    p.g1 = GemmJob<T>{g, Eprev, A, ws.T1, ws.splitk, ws.splitk_elems};
+    p.wm = WmixArgs<T>{ws.T1, lr, lr, 2 * Din, ws.U, 2, Dout, W->rowptr, W->col, W->val, nullptr};
     // Rnew[(a,w)][a*] = sum_{(p,y)} U[(p,y)][(a,w)] conj(A[p][a*][y])
     GemmDesc h = gemm_desc(chil * Dout, chil, 2 * chir);
     h.K2 = chir;
     h.a_m = 1; h.a_k2 = (long)chil * Dout; h.a_k1 = (long)chir * chil * Dout;
     h.b_k1 = lr; h.b_k2 = 1; h.b_n = chir; h.conj_b = 1;
     h.c_m = chil; h.c_n = 1;
-    gemm_t<T>(h, ws.U, A, Enew, s, ws);
+    p.g3 = GemmJob<T>{h, ws.U, A, Enew, ws.splitk, ws.splitk_elems};
   }
+  return p;
 }
 
+template <typename T>
+void run_plan(const ContractPlan<T>& p, hipStream_t s, const ContractWs<T>& ws) {
+  gemm_t<T>(p.g1.d, p.g1.A, p.g1.B, p.g1.C, s, ws);
+  if (p.mix) {
+    WSparseDev<T> w;
+    w.rowptr = const_cast<int*>(p.wm.rowptr);
+    w.col = const_cast<int*>(p.wm.col);
+    w.val = const_cast<cplx<T>*>(p.wm.val);
+    launch_wmix<T>(p.wm.Tin, p.wm.in_stride, p.wm.nxc, p.wm.nin, p.wm.U, p.wm.nq, p.wm.Dout, w, s, p.wm.skip);
+  }
+  gemm_t<T>(p.g3.d, p.g3.A, p.g3.B, p.g3.C, s, ws);
+}
+
+template <typename T>
+void heff_apply(int kind, const cplx<T>* x, cplx<T>* y, const cplx<T>* L, const cplx<T>* R,
+                const WSparseDev<T>* W, int Din, int Dout, int chil, int chir,
+                const ContractWs<T>& ws, const int* skip, hipStream_t s) {
+  run_plan<T>(heff_plan<T>(kind, x, y, L, R, W, Din, Dout, chil, chir, ws, skip), s, ws);
+}
+
+template <typename T>
+void env_update(bool left, const cplx<T>* A, const cplx<T>* Eprev, cplx<T>* Enew,
+                const WSparseDev<T>* W, int Din, int Dout, int chil, int chir,
+                const ContractWs<T>& ws, hipStream_t s) {
+  run_plan<T>(env_plan<T>(left, A, Eprev, Enew, W, Din, Dout, chil, chir, ws), s, ws);
+}
+
+template ContractPlan<double> heff_plan<double>(int, const c128*, c128*, const c128*, const c128*,
+                                                const WSparseDev<double>*, int, int, int, int,
+                                                const ContractWs<double>&, const int*);
+template ContractPlan<float> heff_plan<float>(int, const c64*, c64*, const c64*, const c64*,
+                                              const WSparseDev<float>*, int, int, int, int,
+                                              const ContractWs<float>&, const int*);
+template ContractPlan<double> env_plan<double>(bool, const c128*, const c128*, c128*,
+                                               const WSparseDev<double>*, int, int, int, int,
+                                               const ContractWs<double>&);
+template ContractPlan<float> env_plan<float>(bool, const c64*, const c64*, c64*, const WSparseDev<float>*,
+                                             int, int, int, int, const ContractWs<float>&);
 template void heff_apply<double>(int, const c128*, c128*, const c128*, const c128*,
                                  const WSparseDev<double>*, int, int, int, int,
                                  const ContractWs<double>&, const int*, hipStream_t);
@@ -178,12 +223,23 @@ std::vector<c128> merge_mpo(const c128* Wl, const c128* Wr, int Dl, int Dm, int 
   return out;
 }
 
+namespace {
+size_t sparse_alloc_size(size_t bytes) { return bytes < 4 ? 4 : bytes; }
+}  // namespace
+
+size_t sparse_device_bytes(const HostSparse& h) {
+  const size_t nout = h.rowptr.size() - 1, nnz = h.col.size();
+  return sparse_alloc_size(sizeof(int) * (nout + 1)) + sparse_alloc_size(sizeof(int) * nnz) +
+         sparse_alloc_size(sizeof(c128) * nnz) + sparse_alloc_size(sizeof(c64) * nnz);
+}
+
 void WSparsePair::upload(const HostSparse& h, int nin) {
   const int nout = (int)h.rowptr.size() - 1, nnz = (int)h.col.size();
   auto alloc = [&](size_t bytes) {
     void* p = nullptr;
-    QCA_HIP(hipMalloc(&p, bytes < 4 ? 4 : bytes));
+    QCA_HIP(hipMalloc(&p, sparse_alloc_size(bytes)));
     owned.push_back(p);
+    this->bytes += sparse_alloc_size(bytes);
     return p;
   };
   d64.rowptr = (int*)alloc(sizeof(int) * (nout + 1));
@@ -208,6 +264,7 @@ void WSparsePair::upload(const HostSparse& h, int nin) {
 void WSparsePair::release() {
   for (void* p : owned) (void)hipFree(p);
   owned.clear();
+  bytes = 0;
 }
 
 }  // namespace qca

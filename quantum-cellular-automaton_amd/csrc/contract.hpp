@@ -10,7 +10,7 @@
 
 #include <vector>
 
-#include "kernels.hpp"
+#include "grouped.hpp"
 
 namespace qca {
 
@@ -27,6 +27,25 @@ void gemm_t(const GemmDesc& d, const cplx<T>* A, const cplx<T>* B, cplx<T>* C, h
             const ContractWs<T>& ws);
 
 GemmDesc gemm_desc(int M, int N, int K);
+
+// A contraction as its launches: GEMM over the first environment, sparse MPO
+// mixing (absent for the bond matrix), GEMM over the second operand.  The
+// single-instance launchers run it; the grouped engine batches the same plans.
+template <typename T>
+struct ContractPlan {
+  GemmJob<T> g1;
+  bool mix;
+  WmixArgs<T> wm;
+  GemmJob<T> g3;
+};
+template <typename T>
+ContractPlan<T> heff_plan(int kind, const cplx<T>* x, cplx<T>* y, const cplx<T>* L, const cplx<T>* R,
+                          const WSparseDev<T>* W, int Din, int Dout, int chil, int chir,
+                          const ContractWs<T>& ws, const int* skip);
+template <typename T>
+ContractPlan<T> env_plan(bool left, const cplx<T>* A, const cplx<T>* Eprev, cplx<T>* Enew,
+                         const WSparseDev<T>* W, int Din, int Dout, int chil, int chir,
+                         const ContractWs<T>& ws);
 
 // kind 0: 2-site (W = merged, dq = 4), kind 1: 1-site (dq = 2), kind 2: bond.
 // Din = MPO dimension of L, Dout = MPO dimension of R (kind 2: Din == Dout).
@@ -57,8 +76,11 @@ struct WSparsePair {
   WSparseDev<double> d64;
   WSparseDev<float> d32;
   std::vector<void*> owned;
+  size_t bytes = 0;   // device bytes of the uploaded form
   void upload(const HostSparse& h, int nin);
   void release();
 };
+// Device bytes WSparsePair::upload allocates for h.
+size_t sparse_device_bytes(const HostSparse& h);
 
 }  // namespace qca

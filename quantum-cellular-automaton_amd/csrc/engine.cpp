@@ -42,15 +42,56 @@ template <typename T>
 T* Engine::dmalloc(long n) {
   void* p = nullptr;
   if (n <= 0) n = 1;
+  device_bytes_ += sizeof(T) * (size_t)n;
+  if (count_only_) return nullptr;
   QCA_HIP(hipMalloc(&p, sizeof(T) * (size_t)n));
   allocs_.push_back(p);
   return reinterpret_cast<T*>(p);
+}
+
+// Uploads a sparse MPO form (or, when only counting, adds its size).
+void Engine::upload_sparse(WSparsePair& dst, const HostSparse& h, int nin) {
+  if (count_only_) {
+    device_bytes_ += sparse_device_bytes(h);
+    return;
+  }
+  dst.upload(h, nin);
+  device_bytes_ += dst.bytes;
+}
+
+size_t Engine::footprint(const EngineParams& p, const std::vector<const c128*>& W_host,
+                         const std::vector<int>& Wdl, const std::vector<int>& Wdr,
+                         const std::vector<int>& chis) {
+  Engine e(p, W_host, Wdl, Wdr, chis);
+  return e.device_bytes_;
+}
+
+Engine::Engine(const EngineParams& p, const std::vector<const c128*>& W_host,
+               const std::vector<int>& Wdl, const std::vector<int>& Wdr, const std::vector<int>& chis)
+    : p_(p), N_(p.ncells), device_(-1), count_only_(true) {
+  const std::vector<const c128*> no_state(N_, nullptr);
+  init(W_host, Wdl, Wdr, no_state, chis);
 }
 
 Engine::Engine(int device, const EngineParams& p, const std::vector<const c128*>& W_host,
                const std::vector<int>& Wdl, const std::vector<int>& Wdr,
                const std::vector<const c128*>& A_host, const std::vector<int>& chis)
     : p_(p), N_(p.ncells), device_(device) {
+  // A constructor that throws never reaches the destructor: release whatever
+  // was allocated before the failure, then rethrow.
+  try {
+    init(W_host, Wdl, Wdr, A_host, chis);
+  } catch (...) {
+    release_all();
+    throw;
+  }
+}
+
+void Engine::init(const std::vector<const c128*>& W_host, const std::vector<int>& Wdl,
+                  const std::vector<int>& Wdr, const std::vector<const c128*>& A_host,
+                  const std::vector<int>& chis) {
+  const EngineParams& p = p_;
+  const int device = device_;
   QCA_CHECK(N_ >= 2, "at least 2 cells are required");
   QCA_CHECK((int)W_host.size() == N_ && (int)Wdl.size() == N_ && (int)Wdr.size() == N_,
             "one MPO tensor per cell");
@@ -63,9 +104,11 @@ Engine::Engine(int device, const EngineParams& p, const std::vector<const c128*>
   QCA_CHECK(Wdl[0] == 1 && Wdr[N_ - 1] == 1, "MPO boundary bonds must be 1");
   for (int i = 0; i + 1 < N_; ++i) QCA_CHECK(Wdr[i] == Wdl[i + 1], "MPO bond mismatch");
   for (int i = 0; i < N_; ++i) QCA_CHECK(chis[i] >= 1 && chis[i] <= 256, "MPS bond out of range");
-  QCA_HIP(hipSetDevice(device));
-  QCA_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
-  own_stream_ = true;
+  if (!count_only_) {
+    QCA_HIP(hipSetDevice(device));
+    QCA_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+    own_stream_ = true;
+  }
 
   chi_ = chis;
   cap_.resize(N_ + 1);
@@ -85,14 +128,14 @@ Engine::Engine(int device, const EngineParams& p, const std::vector<const c128*>
   Wf_.resize(N_);
   Wb_.resize(N_);
   for (int i = 0; i < N_; ++i) {
-    Wf_[i].upload(build_sparse(W_host[i], 2, Dl_[i], Dr_[i], false), 2 * Dl_[i]);
-    Wb_[i].upload(build_sparse(W_host[i], 2, Dl_[i], Dr_[i], true), 2 * Dr_[i]);
+    upload_sparse(Wf_[i], build_sparse(W_host[i], 2, Dl_[i], Dr_[i], false), 2 * Dl_[i]);
+    upload_sparse(Wb_[i], build_sparse(W_host[i], 2, Dl_[i], Dr_[i], true), 2 * Dr_[i]);
   }
   if (p.algorithm != 1) {  // two-site blocks: 2TDVP and the adaptive targets
     W2f_.resize(N_ - 1);
     for (int s = 0; s + 1 < N_; ++s) {
       auto w2 = merge_mpo(W_host[s], W_host[s + 1], Dl_[s], Dr_[s], Dr_[s + 1]);
-      W2f_[s].upload(build_sparse(w2.data(), 4, Dl_[s], Dr_[s + 1], false), 4 * Dl_[s]);
+      upload_sparse(W2f_[s], build_sparse(w2.data(), 4, Dl_[s], Dr_[s + 1], false), 4 * Dl_[s]);
     }
   }
 
@@ -102,8 +145,9 @@ Engine::Engine(int device, const EngineParams& p, const std::vector<const c128*>
   A_.resize(N_);
   for (int i = 0; i < N_; ++i) {
     A_[i] = dmalloc<c128>(2L * cap_[i] * cap_[i + 1]);
-    QCA_HIP(hipMemcpy(A_[i], A_host[i], sizeof(c128) * 2L * chi_[i] * chi_[i + 1],
-                      hipMemcpyHostToDevice));
+    if (!count_only_)
+      QCA_HIP(hipMemcpy(A_[i], A_host[i], sizeof(c128) * 2L * chi_[i] * chi_[i + 1],
+                        hipMemcpyHostToDevice));
   }
   L_.assign(N_, nullptr);
   R_.assign(N_, nullptr);
@@ -113,7 +157,7 @@ Engine::Engine(int device, const EngineParams& p, const std::vector<const c128*>
   }
   one64_ = dmalloc<c128>(1);
   one32_ = dmalloc<c64>(1);
-  {
+  if (!count_only_) {
     c128 o{1.0, 0.0};
     c64 o32{1.0f, 0.0f};
     QCA_HIP(hipMemcpy(one64_, &o, sizeof(o), hipMemcpyHostToDevice));
@@ -165,7 +209,7 @@ Engine::Engine(int device, const EngineParams& p, const std::vector<const c128*>
                                          (int)std::min<long>(2 * cmax, 512));
     jw_.pre = dmalloc<c128>((long)jw_.pre_elems);
   }
-  QCA_HIP(hipHostMalloc(&jw_.host, 64));
+  if (!count_only_) QCA_HIP(hipHostMalloc(&jw_.host, 64));
   if (mixed) {
     jw_.tol = MIXED_SVD_TOL;
     if (const char* env = std::getenv("QCA_MIXED_SVD_TOL")) {
@@ -175,10 +219,12 @@ Engine::Engine(int device, const EngineParams& p, const std::vector<const c128*>
       jw_.tol = v;
     }
   }
-  QCA_HIP(hipHostMalloc((void**)&svdres_host_, sizeof(SvdResult)));
-  QCA_HIP(hipHostMalloc((void**)&ctrl_host_, sizeof(LanczosCtrl)));
-  QCA_HIP(hipEventCreate(&ev0_));
-  QCA_HIP(hipEventCreate(&ev1_));
+  if (!count_only_) {
+    QCA_HIP(hipHostMalloc((void**)&svdres_host_, sizeof(SvdResult)));
+    QCA_HIP(hipHostMalloc((void**)&ctrl_host_, sizeof(LanczosCtrl)));
+    QCA_HIP(hipEventCreate(&ev0_));
+    QCA_HIP(hipEventCreate(&ev1_));
+  }
   lw_.max_n = nmax;
   lw_.max_blocks = lanczos_nblk(nmax);
   lw_.ctrl = dmalloc<LanczosCtrl>(1);
@@ -190,7 +236,7 @@ Engine::Engine(int device, const EngineParams& p, const std::vector<const c128*>
   lw_.coef1 = dmalloc<c128>(LZ_MAXK);
   lw_.coef2 = dmalloc<c128>(LZ_MAXK);
   tally_ = dmalloc<KrylovTally>(1);
-  QCA_HIP(hipMemset(tally_, 0, sizeof(KrylovTally)));
+  if (!count_only_) QCA_HIP(hipMemset(tally_, 0, sizeof(KrylovTally)));
   fused_max_n_ = FUSED_KRYLOV_DEFAULT_N;
   if (const char* env = std::getenv("QCA_FUSED_KRYLOV_MAX_N")) {
     char* end = nullptr;
@@ -203,6 +249,7 @@ Engine::Engine(int device, const EngineParams& p, const std::vector<const c128*>
   measT_ = dmalloc<c128>(2 * cmax * cmax);
   rho_ = dmalloc<c128>(4L * N_);
   obs_ = dmalloc<double>(3L * N_);
+  if (count_only_) return;
 
   // ---- TDVP construction (reference tdvp.py:20-39): canonicalise, then the
   // right environments of every site.
@@ -211,18 +258,31 @@ Engine::Engine(int device, const EngineParams& p, const std::vector<const c128*>
   sync();
 }
 
-Engine::~Engine() {
+Engine::~Engine() { release_all(); }
+
+// Frees everything the engine holds; safe on a partly constructed engine and
+// when called twice.
+void Engine::release_all() {
+  if (count_only_) return;
+  (void)hipSetDevice(device_);
   if (s_) (void)hipStreamSynchronize(s_);
   for (auto& w : Wf_) w.release();
   for (auto& w : Wb_) w.release();
   for (auto& w : W2f_) w.release();
   for (void* p : allocs_) (void)hipFree(p);
+  allocs_.clear();
   if (svdres_host_) (void)hipHostFree(svdres_host_);
   if (ctrl_host_) (void)hipHostFree(ctrl_host_);
   if (jw_.host) (void)hipHostFree(jw_.host);
+  svdres_host_ = nullptr;
+  ctrl_host_ = nullptr;
+  jw_.host = nullptr;
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
+  ev0_ = ev1_ = nullptr;
   if (own_stream_ && s_) (void)hipStreamDestroy(s_);
+  s_ = nullptr;
+  own_stream_ = false;
 }
 
 void Engine::sync() { QCA_HIP(hipStreamSynchronize(s_)); }
@@ -357,7 +417,7 @@ void Engine::lanczos_run(int kind, int site, c128* x, long n, int chil, int chir
   int j = 0;
   int chunk = std::max(2, kpred_[kind]);
   while (true) {
-    for (int c = 0; c < chunk && j < lw_.max_m - 2; ++c, ++j) {
+    for (int c = 0; c < chunk && j < lw_.max_m - 1; ++c, ++j) {
       if constexpr (std::is_same<T, double>::value) {
         heff<double>(kind, site, lw_.V + (long)j * n, y64_, chil, chir);
         lanczos_iteration_tail<double>(y64_, n, j, tau, p_.krylov_tol, lw_, s_);
@@ -370,7 +430,9 @@ void Engine::lanczos_run(int kind, int site, c128* x, long n, int chil, int chir
     QCA_HIP(hipEventRecord(ev1_, s_));
     sync();
     if (ctrl_host_->done) break;
-    QCA_CHECK(j < lw_.max_m - 2, "Krylov exponential did not converge");
+    // the device closes the exponential at the dimension limit (m = max_m - 1)
+    // at the latest, so this only fires on non-finite input
+    QCA_CHECK(j < lw_.max_m - 1, "Krylov exponential did not converge (non-finite input?)");
     chunk = 2;
   }
   stats_.krylov_ms += elapsed_ms();
@@ -425,13 +487,45 @@ void Engine::update_env(bool left, int site) {
 }
 
 // ---------------------------------------------------------------- 2-site split
-void Engine::merge_theta(int s, int chil, int chim, int chir) {
+// Theta[p1][p2][a][c] = sum_b A_s[p1][a][b] A_{s+1}[p2][b][c]  (reference tdvp.py:278-280)
+GemmDesc merge_theta_desc(int chil, int chim, int chir) {
   GemmDesc g = gemm_desc(chil, chir, chim);
   g.NB1 = 2; g.NB2 = 2;
   g.a_m = chim; g.a_k2 = 1; g.a_b1 = (long)chil * chim;
   g.b_k2 = chir; g.b_n = 1; g.b_b2 = (long)chim * chir;
   g.c_m = chir; g.c_n = 1; g.c_b1 = 2L * chil * chir; g.c_b2 = (long)chil * chir;
-  gemm<double>(g, A_[s], A_[s + 1], theta_);
+  return g;
+}
+
+// The factor that takes the (normalised) singular values after the split
+// (reference tdvp.py:293-296), as a GEMM of the gathered isometry with Theta.
+// right_sweep: A[s+1][p2][k][c] = sum_{(p1,a)} conj(A[s][(p1,a)][k]) Theta[p1][p2][a][c] / |s_k|
+//              (A = A[s], B = Theta, C = A[s+1]);
+// else:        A[s][p1][a][k] = sum_{(p2,c)} Theta[p1][p2][a][c] conj(A[s+1][p2][k][c]) / |s_k|
+//              (A = Theta, B = A[s+1], C = A[s]).
+GemmDesc split_theta_desc(bool right_sweep, int chil, int chir, int k, double inv_norm) {
+  if (right_sweep) {
+    GemmDesc g = gemm_desc(k, chir, 2 * chil);
+    g.K2 = chil;
+    g.NB2 = 2;
+    g.a_m = 1; g.a_k1 = (long)chil * k; g.a_k2 = k; g.conj_a = 1;
+    g.b_k1 = 2L * chil * chir; g.b_k2 = chir; g.b_n = 1; g.b_b2 = (long)chil * chir;
+    g.c_m = chir; g.c_n = 1; g.c_b2 = (long)k * chir;
+    g.alpha_re = inv_norm;
+    return g;
+  }
+  GemmDesc g = gemm_desc(chil, k, 2 * chir);
+  g.K2 = chir;
+  g.NB2 = 2;
+  g.a_m = chir; g.a_k1 = (long)chil * chir; g.a_k2 = 1; g.a_b2 = 2L * chil * chir;
+  g.b_k1 = (long)k * chir; g.b_k2 = 1; g.b_n = chir; g.conj_b = 1;
+  g.c_m = k; g.c_n = 1; g.c_b2 = (long)chil * k;
+  g.alpha_re = inv_norm;
+  return g;
+}
+
+void Engine::merge_theta(int s, int chil, int chim, int chir) {
+  gemm<double>(merge_theta_desc(chil, chim, chir), A_[s], A_[s + 1], theta_);
 }
 
 int Engine::split_theta(int s, bool right_sweep, int chil, int chir) {
@@ -448,33 +542,19 @@ int Engine::split_theta(int s, bool right_sweep, int chil, int chir) {
   stats_.svd_calls++;
   const int k = svdres_host_->k;
   stats_.svd_sweeps += svdres_host_->nsweeps;
+  if (!svdres_host_->converged)
+    throw Error(QCA_ERR_NOT_CONVERGED_CODE, "Jacobi SVD of the two-site split did not converge");
   QCA_CHECK(k >= 1, "SVD truncation produced an empty bond");
   QCA_CHECK(k <= cap_[s + 1], "truncation rank above bond capacity");
   const double inv = 1.0 / svdres_host_->norm_kept;
   if (right_sweep) {
     // A[s][p][a][k] = U[(p,a)][k]
     launch_gather_isometry(svdX_, perm_, sigma_, A_[s], rows, chil, k, 0, s_);
-    // A[s+1][p2][k][c] = sum_{(p1,a)} conj(A[s][(p1,a)][k]) Theta[p1][p2][a][c] / |s_k|
-    GemmDesc g = gemm_desc(k, chir, 2 * chil);
-    g.K2 = chil;
-    g.NB2 = 2;
-    g.a_m = 1; g.a_k1 = (long)chil * k; g.a_k2 = k; g.conj_a = 1;
-    g.b_k1 = 2L * chil * chir; g.b_k2 = chir; g.b_n = 1; g.b_b2 = (long)chil * chir;
-    g.c_m = chir; g.c_n = 1; g.c_b2 = (long)k * chir;
-    g.alpha_re = inv;
-    gemm<double>(g, A_[s], theta_, A_[s + 1]);
+    gemm<double>(split_theta_desc(true, chil, chir, k, inv), A_[s], theta_, A_[s + 1]);
   } else {
     // A[s+1][p2][k][c] = conj(V[(p2,c)][k])
     launch_gather_isometry(svdX_, perm_, sigma_, A_[s + 1], rows, chir, k, 1, s_);
-    // A[s][p1][a][k] = sum_{(p2,c)} Theta[p1][p2][a][c] conj(A[s+1][p2][k][c]) / |s_k|
-    GemmDesc g = gemm_desc(chil, k, 2 * chir);
-    g.K2 = chir;
-    g.NB2 = 2;
-    g.a_m = chir; g.a_k1 = (long)chil * chir; g.a_k2 = 1; g.a_b2 = 2L * chil * chir;
-    g.b_k1 = (long)k * chir; g.b_k2 = 1; g.b_n = chir; g.conj_b = 1;
-    g.c_m = k; g.c_n = 1; g.c_b2 = (long)chil * k;
-    g.alpha_re = inv;
-    gemm<double>(g, theta_, A_[s + 1], A_[s]);
+    gemm<double>(split_theta_desc(false, chil, chir, k, inv), theta_, A_[s + 1], A_[s]);
   }
   chi_[s + 1] = k;
   return k;
@@ -667,6 +747,8 @@ void Engine::compute_targets() {
     QCA_HIP(hipMemcpyAsync(svdres_host_, svdres_, sizeof(SvdResult), hipMemcpyDeviceToHost, s_));
     sync();
     stats_.svd_sweeps += svdres_host_->nsweeps;
+    if (!svdres_host_->converged)
+      throw Error(QCA_ERR_NOT_CONVERGED_CODE, "Jacobi SVD of the adaptive bond target did not converge");
     target_[bond] = std::max(1, svdres_host_->k);
     qr_right_site(bond, chi_[bond], cbond_, cr, cc, false);
     absorb_right(bond - 1, cbond_, cr, cc);

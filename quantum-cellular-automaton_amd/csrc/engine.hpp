@@ -42,13 +42,30 @@ struct StepStats {
   double krylov_ms = 0;    // device time of the Krylov exponentials
 };
 
+// GEMM of the two-site merge and of the split's second factor (shared by the
+// engine and the grouped stepping of many engines).
+GemmDesc merge_theta_desc(int chil, int chim, int chir);
+GemmDesc split_theta_desc(bool right_sweep, int chil, int chir, int k, double inv_norm);
+
+class Group;
+
 class Engine {
+  friend class Group;
+
  public:
   Engine(int device, const EngineParams& p, const std::vector<const c128*>& W_host,
          const std::vector<int>& Wdl, const std::vector<int>& Wdr,
          const std::vector<const c128*>& A_host, const std::vector<int>& chis);
   ~Engine();
   Engine(const Engine&) = delete;
+  // Device bytes an engine of these parameters allocates (MPS, environments,
+  // sparse MPO forms, every workspace), computed by running the constructor's
+  // sizing without touching a device.
+  static size_t footprint(const EngineParams& p, const std::vector<const c128*>& W_host,
+                          const std::vector<int>& Wdl, const std::vector<int>& Wdr,
+                          const std::vector<int>& chis);
+  // Device bytes this engine holds (the same sum, from its allocations).
+  size_t device_bytes() const { return device_bytes_; }
   Engine& operator=(const Engine&) = delete;
 
   void step();                                        // one time step (both half sweeps)
@@ -72,6 +89,16 @@ class Engine {
   void set_fused_max_n(long n) { fused_max_n_ = n; }
 
  private:
+  // counting constructor behind footprint(): sizes only, no HIP call
+  Engine(const EngineParams& p, const std::vector<const c128*>& W_host, const std::vector<int>& Wdl,
+         const std::vector<int>& Wdr, const std::vector<int>& chis);
+  void upload_sparse(WSparsePair& dst, const HostSparse& h, int nin);
+  bool count_only_ = false;
+  size_t device_bytes_ = 0;
+  void init(const std::vector<const c128*>& W_host, const std::vector<int>& Wdl,
+            const std::vector<int>& Wdr, const std::vector<const c128*>& A_host,
+            const std::vector<int>& chis);
+  void release_all();
   // ---------------- helpers
   template <typename T> T* Lenv(int site) const;       // L[site], site >= -1
   template <typename T> T* Renv(int site) const;       // R[site], site <= N

@@ -11,7 +11,7 @@
 // reads stay coalesced; the next K-tile is loaded into registers while the
 // current one is multiplied.  Launches with too few output tiles to fill 256
 // CUs split K and reduce the partial tiles in a second, deterministic pass.
-#include "qca_common.hpp"
+#include "grouped.hpp"
 
 namespace qca {
 namespace {
@@ -48,12 +48,14 @@ struct Mfma<float> {
 // groups' partial tiles are summed through LDS at the end.  With KG = 2 a CU
 // holding one workgroup runs two waves per SIMD, so one group's staging and
 // barrier waits overlap the other's MFMAs.
+// One output tile (bx: N tile, by: M tile, bz: batch * splits + split) of the
+// GEMM d; the kernels below supply the block index (direct launch) or a
+// record's virtual one (grouped launch over many instances).
 template <typename T, bool A_MCONT, bool B_NCONT, int KG>
-__global__ __launch_bounds__(256 * KG) void gemm_kernel(GemmDesc d, const cplx<T>* __restrict__ A,
-                                                        const cplx<T>* __restrict__ B,
-                                                        cplx<T>* __restrict__ C,
-                                                        cplx<T>* __restrict__ ws, int splits,
-                                                        int k_per_split) {
+__device__ __forceinline__ void gemm_body(const GemmDesc& d, const cplx<T>* __restrict__ A,
+                                          const cplx<T>* __restrict__ B, cplx<T>* __restrict__ C,
+                                          cplx<T>* __restrict__ ws, int splits, int k_per_split,
+                                          unsigned bx, unsigned by, unsigned bz) {
   using acc_t = typename Mfma<T>::acc_t;
   static_assert(BM == BN, "one LDS stride for both operands");
   if (d.skip && *d.skip) return;
@@ -71,10 +73,10 @@ __global__ __launch_bounds__(256 * KG) void gemm_kernel(GemmDesc d, const cplx<T
   const int wave = tid >> 6;
   const int wm = (wave >> 1) * 32;
   const int wn = (wave & 1) * 32;
-  const int m0 = blockIdx.y * BM;
-  const int n0 = blockIdx.x * BN;
-  const int zb = blockIdx.z / splits;
-  const int split = blockIdx.z - zb * splits;
+  const int m0 = by * BM;
+  const int n0 = bx * BN;
+  const int zb = bz / splits;
+  const int split = bz - zb * splits;
   const int b1 = zb / d.NB2;
   const int b2 = zb - b1 * d.NB2;
 
@@ -236,7 +238,7 @@ __global__ __launch_bounds__(256 * KG) void gemm_kernel(GemmDesc d, const cplx<T
         if (m >= d.M || n >= d.N) continue;
         const T xr = acc_re[i][j][r], xi = acc_im[i][j][r];
         if (splits > 1) {
-          ws[((long)blockIdx.z * d.M + m) * d.N + n] = cplx<T>{xr, xi};
+          ws[((long)bz * d.M + m) * d.N + n] = cplx<T>{xr, xi};
         } else {
           cplx<T>* cp = C + b1 * d.c_b1 + b2 * d.c_b2 + m * d.c_m + n * d.c_n;
           cplx<T> out{al_re * xr - al_im * xi, al_re * xi + al_im * xr};
@@ -249,16 +251,28 @@ __global__ __launch_bounds__(256 * KG) void gemm_kernel(GemmDesc d, const cplx<T
       }
 }
 
+template <typename T, bool A_MCONT, bool B_NCONT, int KG>
+__global__ __launch_bounds__(256 * KG) void gemm_kernel(GemmDesc d, const cplx<T>* __restrict__ A,
+                                                        const cplx<T>* __restrict__ B,
+                                                        cplx<T>* __restrict__ C,
+                                                        cplx<T>* __restrict__ ws, int splits,
+                                                        int k_per_split) {
+  gemm_body<T, A_MCONT, B_NCONT, KG>(d, A, B, C, ws, splits, k_per_split, blockIdx.x, blockIdx.y,
+                                     blockIdx.z);
+}
+
 // Sum the split-K partial tiles in split order (deterministic) and apply
-// alpha/beta while scattering into C's strides.
+// alpha/beta while scattering into C's strides; block `blk` of `nblk`, 256
+// threads each.
 template <typename T>
-__global__ void splitk_reduce(GemmDesc d, const cplx<T>* __restrict__ ws, cplx<T>* __restrict__ C,
-                              int splits, int batch) {
+__device__ __forceinline__ void splitk_reduce_body(const GemmDesc& d, const cplx<T>* __restrict__ ws,
+                                                   cplx<T>* __restrict__ C, int splits, int batch,
+                                                   unsigned blk, unsigned nblk) {
   if (d.skip && *d.skip) return;
   const long mn = (long)d.M * d.N;
   const long total = mn * batch;
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
-       idx += (long)gridDim.x * blockDim.x) {
+  for (long idx = blk * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)nblk * blockDim.x) {
     const int zb = (int)(idx / mn);
     const long r = idx - zb * mn;
     const int m = (int)(r / d.N), n = (int)(r - (long)m * d.N);
@@ -284,20 +298,39 @@ __global__ void splitk_reduce(GemmDesc d, const cplx<T>* __restrict__ ws, cplx<T
 }
 
 template <typename T>
-void gemm_launch(const GemmDesc& d, const cplx<T>* A, const cplx<T>* B, cplx<T>* C,
-                 hipStream_t s, cplx<T>* ws, long ws_elems) {
+__global__ void splitk_reduce(GemmDesc d, const cplx<T>* __restrict__ ws, cplx<T>* __restrict__ C,
+                              int splits, int batch) {
+  splitk_reduce_body<T>(d, ws, C, splits, batch, blockIdx.x, gridDim.x);
+}
+
+// Launch decisions of one GEMM, a function of its own descriptor and
+// workspace only (so a GEMM computes the same whether launched alone or
+// grouped with others).
+struct GemmPlan {
+  bool empty = false;
+  int tm = 0, tn = 0, batch = 1, splits = 1, kps = 0;
+  bool am = false, bn = false, kg2 = false;
+  int reduce_blocks = 0;
+};
+
+GemmPlan gemm_plan(const GemmDesc& d, long ws_elems) {
   QCA_CHECK(d.M >= 0 && d.N >= 0 && d.K >= 0, "negative GEMM extent");
   QCA_CHECK(d.K2 >= 1 && (d.K == 0 || d.K % d.K2 == 0), "K2 must divide K");
   QCA_CHECK(d.NB1 >= 1 && d.NB2 >= 1, "empty batch");
-  const int batch = d.NB1 * d.NB2;
-  if (d.M == 0 || d.N == 0) return;
-  const int tm = ceil_div(d.M, BM), tn = ceil_div(d.N, BN);
-  const long tiles = (long)tm * tn * batch;
+  GemmPlan p;
+  p.batch = d.NB1 * d.NB2;
+  if (d.M == 0 || d.N == 0) {
+    p.empty = true;
+    return p;
+  }
+  p.tm = ceil_div(d.M, BM);
+  p.tn = ceil_div(d.N, BN);
+  const long tiles = (long)p.tm * p.tn * p.batch;
   // split K when the output tiles alone cannot occupy the chip
   int splits = 1;
   if (tiles < 192 && d.K >= 4 * BK) {
     splits = (int)std::min<long>((256 + tiles - 1) / tiles, d.K / (2 * BK));
-    while (splits > 1 && (long)splits * batch * d.M * d.N > ws_elems) --splits;
+    while (splits > 1 && (long)splits * p.batch * d.M * d.N > ws_elems) --splits;
     if (splits < 1) splits = 1;
   }
   int kps = d.K;
@@ -311,11 +344,28 @@ void gemm_launch(const GemmDesc& d, const cplx<T>* A, const cplx<T>* B, cplx<T>*
     kps = 0;
   }
   QCA_CHECK((long)tiles * splits < (1L << 31), "GEMM grid too large");
-  QCA_CHECK(batch * splits <= 65535, "GEMM batch*splits exceeds grid z limit");
-  dim3 grid(tn, tm, batch * splits);
-  const bool am = (d.a_m == 1), bn = (d.b_n == 1);
+  QCA_CHECK(p.batch * splits <= 65535, "GEMM batch*splits exceeds grid z limit");
+  p.splits = splits;
+  p.kps = kps;
+  p.am = (d.a_m == 1);
+  p.bn = (d.b_n == 1);
   // two K-groups per workgroup when each has at least two K-tiles to take
-  const bool kg2 = kps >= 4 * BK;
+  p.kg2 = kps >= 4 * BK;
+  if (splits > 1) {
+    const long total = (long)d.M * d.N * p.batch;
+    p.reduce_blocks = (int)std::min<long>(ceil_div(total, 256), 4096);
+  }
+  return p;
+}
+
+template <typename T>
+void gemm_launch(const GemmDesc& d, const cplx<T>* A, const cplx<T>* B, cplx<T>* C,
+                 hipStream_t s, cplx<T>* ws, long ws_elems) {
+  const GemmPlan p = gemm_plan(d, ws_elems);
+  if (p.empty) return;
+  const int splits = p.splits, kps = p.kps;
+  dim3 grid(p.tn, p.tm, p.batch * splits);
+  const bool am = p.am, bn = p.bn, kg2 = p.kg2;
 #define QCA_GEMM_L(AM, BNC)                                                                     \
   do {                                                                                         \
     if (kg2)                                                                                   \
@@ -332,14 +382,93 @@ void gemm_launch(const GemmDesc& d, const cplx<T>* A, const cplx<T>* B, cplx<T>*
 #undef QCA_GEMM_L
   QCA_HIP(hipGetLastError());
   if (splits > 1) {
-    const long total = (long)d.M * d.N * batch;
-    const int blocks = (int)std::min<long>(ceil_div(total, 256), 4096);
-    hipLaunchKernelGGL(splitk_reduce<T>, dim3(blocks), dim3(256), 0, s, d, ws, C, splits, batch);
+    hipLaunchKernelGGL(splitk_reduce<T>, dim3(p.reduce_blocks), dim3(256), 0, s, d, ws, C, splits,
+                       p.batch);
     QCA_HIP(hipGetLastError());
   }
 }
 
+// ------------------------------------------------------------------ grouped
+template <typename T, bool A_MCONT, bool B_NCONT, int KG>
+__global__ __launch_bounds__(256 * KG) void gemm_grouped_kernel(const GRec<GemmArgs<T>>* __restrict__ recs,
+                                                                int n, int j) {
+  unsigned bx, by, bz;
+  const GRec<GemmArgs<T>>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
+  const GemmDesc d = r.a.d;
+  gemm_body<T, A_MCONT, B_NCONT, KG>(d, r.a.A, r.a.B + (long)j * r.a.b_jstride, r.a.C, r.a.ws, r.a.splits,
+                                     r.a.kps, bx, by, bz);
+}
+
+template <typename T>
+__global__ void splitk_reduce_grouped(const GRec<GemmArgs<T>>* __restrict__ recs, int n) {
+  unsigned bx, by, bz;
+  const GRec<GemmArgs<T>>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
+  const GemmDesc d = r.a.d;
+  splitk_reduce_body<T>(d, r.a.ws, r.a.C, r.a.splits, d.NB1 * d.NB2, bx, r.grid.x);
+}
+
 }  // namespace
+
+template <typename T>
+void GemmGroup<T>::prepare(const std::vector<GemmJob<T>>& jobs, RecordArena& arena) {
+  std::vector<GRec<GemmArgs<T>>> var[8], red;
+  for (const auto& j : jobs) {
+    const GemmPlan p = gemm_plan(j.d, j.ws_elems);
+    if (p.empty) continue;
+    GRec<GemmArgs<T>> r{};
+    r.a.d = j.d;
+    r.a.A = j.A;
+    r.a.B = j.B;
+    r.a.C = j.C;
+    r.a.ws = j.ws;
+    r.a.b_jstride = j.b_jstride;
+    r.a.splits = p.splits;
+    r.a.kps = p.kps;
+    r.grid.x = p.tn;
+    r.grid.y = p.tm;
+    r.grid.z = p.batch * p.splits;
+    var[(p.am ? 4 : 0) + (p.bn ? 2 : 0) + (p.kg2 ? 1 : 0)].push_back(r);
+    if (p.splits > 1) {
+      r.grid = VGrid{};
+      r.grid.x = p.reduce_blocks;
+      red.push_back(r);
+    }
+  }
+  for (int v = 0; v < 8; ++v) {
+    launches[v].n = (int)var[v].size();
+    launches[v].blocks = grec_layout(var[v]);
+    launches[v].recs = var[v].empty() ? nullptr : arena.push(var[v]);
+  }
+  reduce.n = (int)red.size();
+  reduce.blocks = grec_layout(red);
+  reduce.recs = red.empty() ? nullptr : arena.push(red);
+}
+
+template <typename T>
+void GemmGroup<T>::launch(hipStream_t s, int j) const {
+#define QCA_GG_L(V, AM, BNC, KG)                                                               \
+  if (launches[V].n)                                                                          \
+    hipLaunchKernelGGL((gemm_grouped_kernel<T, AM, BNC, KG>), dim3(launches[V].blocks),        \
+                       dim3(256 * KG), 0, s, launches[V].recs, launches[V].n, j);
+  QCA_GG_L(0, false, false, 1)
+  QCA_GG_L(1, false, false, 2)
+  QCA_GG_L(2, false, true, 1)
+  QCA_GG_L(3, false, true, 2)
+  QCA_GG_L(4, true, false, 1)
+  QCA_GG_L(5, true, false, 2)
+  QCA_GG_L(6, true, true, 1)
+  QCA_GG_L(7, true, true, 2)
+#undef QCA_GG_L
+  QCA_HIP(hipGetLastError());
+  if (reduce.n) {
+    hipLaunchKernelGGL(splitk_reduce_grouped<T>, dim3(reduce.blocks), dim3(256), 0, s, reduce.recs,
+                       reduce.n);
+    QCA_HIP(hipGetLastError());
+  }
+}
+
+template struct GemmGroup<double>;
+template struct GemmGroup<float>;
 
 void gemm_c128(const GemmDesc& d, const c128* A, const c128* B, c128* C, hipStream_t s,
                c128* ws, long ws_elems) {

@@ -7,7 +7,9 @@
 //                 scaling) so no host round trip is needed per iteration
 //  * measure   -- <n_i> and the single-site density matrix from the transfer
 //  * exact     -- closed-form 2^N rule-Hamiltonian application
-#include "kernels.hpp"
+#include "grouped.hpp"
+
+#include <cstring>
 
 namespace qca {
 namespace {
@@ -61,16 +63,16 @@ constexpr int WMIX_WAVES = 16;
 constexpr int WMIX_DMAX = 16;
 
 template <typename T>
-__global__ __launch_bounds__(WMIX_XC * WMIX_WAVES) void wmix_kernel(
-    const cplx<T>* __restrict__ Tin, long in_stride, long nxc, int nin, cplx<T>* __restrict__ U,
-    int nq, int Dout, const int* __restrict__ rowptr, const int* __restrict__ col,
-    const cplx<T>* __restrict__ val, const int* __restrict__ skip) {
+__device__ __forceinline__ void wmix_body(const cplx<T>* __restrict__ Tin, long in_stride, long nxc,
+                                          int nin, cplx<T>* __restrict__ U, int nq, int Dout,
+                                          const int* __restrict__ rowptr, const int* __restrict__ col,
+                                          const cplx<T>* __restrict__ val, const int* __restrict__ skip,
+                                          unsigned blk, unsigned char* smem_raw) {
   if (skip && *skip) return;
-  extern __shared__ unsigned char smem_raw[];
   cplx<T>* sin = reinterpret_cast<cplx<T>*>(smem_raw);        // [nin][64]
   cplx<T>* sout = sin + (long)nin * WMIX_XC;                   // [nq][WMIX_XC][Dout]
   const int t = threadIdx.x & (WMIX_XC - 1), wv = threadIdx.x / WMIX_XC;
-  const long xc0 = (long)blockIdx.x * WMIX_XC;
+  const long xc0 = (long)blk * WMIX_XC;
   const long xc = xc0 + t;
   const bool live = xc < nxc;
   for (int i = wv; i < nin; i += WMIX_WAVES)
@@ -96,24 +98,56 @@ __global__ __launch_bounds__(WMIX_XC * WMIX_WAVES) void wmix_kernel(
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(WMIX_XC * WMIX_WAVES) void wmix_kernel(
+    const cplx<T>* __restrict__ Tin, long in_stride, long nxc, int nin, cplx<T>* __restrict__ U,
+    int nq, int Dout, const int* __restrict__ rowptr, const int* __restrict__ col,
+    const cplx<T>* __restrict__ val, const int* __restrict__ skip) {
+  extern __shared__ unsigned char smem_raw[];
+  wmix_body<T>(Tin, in_stride, nxc, nin, U, nq, Dout, rowptr, col, val, skip, blockIdx.x, smem_raw);
+}
+
+template <typename T>
+__global__ __launch_bounds__(WMIX_XC * WMIX_WAVES) void wmix_grouped(const GRec<WmixArgs<T>>* __restrict__ recs,
+                                                                     int n) {
+  extern __shared__ unsigned char smem_raw[];
+  unsigned bx, by, bz;
+  const GRec<WmixArgs<T>>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
+  const WmixArgs<T> a = r.a;
+  wmix_body<T>(a.Tin, a.in_stride, a.nxc, a.nin, a.U, a.nq, a.Dout, a.rowptr, a.col, a.val, a.skip, bx,
+               smem_raw);
+}
+
 // ------------------------------------------------------------------ conversions
+template <typename S, typename D>
+__device__ __forceinline__ void convert_body(const cplx<S>* __restrict__ in, cplx<D>* __restrict__ out,
+                                             long n, const int* __restrict__ skip, unsigned blk,
+                                             unsigned nblk) {
+  if (skip && *skip) return;
+  for (long i = blk * (long)blockDim.x + threadIdx.x; i < n; i += (long)nblk * blockDim.x)
+    out[i] = cplx<D>{D(in[i].re), D(in[i].im)};
+}
 template <typename S, typename D>
 __global__ void convert_kernel(const cplx<S>* __restrict__ in, cplx<D>* __restrict__ out, long n,
                                const int* __restrict__ skip) {
-  if (skip && *skip) return;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    out[i] = cplx<D>{D(in[i].re), D(in[i].im)};
+  convert_body<S, D>(in, out, n, skip, blockIdx.x, gridDim.x);
+}
+__global__ void convert_grouped(const GRec<ConvArgs>* __restrict__ recs, int n) {
+  unsigned bx, by, bz;
+  const GRec<ConvArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
+  convert_body<double, float>(r.a.in, r.a.out, r.a.n, r.a.skip, bx, r.grid.x);
 }
 
 // ------------------------------------------------------------------ permutes
 // Theta[p1][p2][a][c] -> column-major X of the SVD matrix M[(p1,a),(p2,c)].
 // mode 0: X[j][r] = M[r][j] (columns of M); mode 1: X[r][j] = conj(M[r][j])
 // (columns of M^dagger).
-__global__ void theta_to_svd(const c128* __restrict__ th, c128* __restrict__ X, int chil, int chir,
-                             int mode) {
+__device__ __forceinline__ void theta_to_svd_body(const c128* __restrict__ th, c128* __restrict__ X,
+                                                  int chil, int chir, int mode, unsigned blk,
+                                                  unsigned nblk) {
   const long m = 2L * chil, n = 2L * chir;
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < m * n;
-       idx += (long)gridDim.x * blockDim.x) {
+  for (long idx = blk * (long)blockDim.x + threadIdx.x; idx < m * n;
+       idx += (long)nblk * blockDim.x) {
     // idx enumerates Theta in memory order: p1, p2, a, c
     const long c = idx % chir;
     long t = idx / chir;
@@ -130,16 +164,26 @@ __global__ void theta_to_svd(const c128* __restrict__ th, c128* __restrict__ X, 
     }
   }
 }
+__global__ void theta_to_svd(const c128* __restrict__ th, c128* __restrict__ X, int chil, int chir,
+                             int mode) {
+  theta_to_svd_body(th, X, chil, chir, mode, blockIdx.x, gridDim.x);
+}
+__global__ void theta_to_svd_grouped(const GRec<ThetaSvdArgs>* __restrict__ recs, int n) {
+  unsigned bx, by, bz;
+  const GRec<ThetaSvdArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
+  theta_to_svd_body(r.a.th, r.a.X, r.a.chil, r.a.chir, r.a.mode, bx, r.grid.x);
+}
 
 // A[p][a][k] = X[perm[k]][(p,a)] / sigma[k]   (mode 0, left isometry from U)
 // A[p][k][c] = conj(X[perm[k]][(p,c)]) / sigma[k]  (mode 1, right isometry from V)
-__global__ void gather_isometry(const c128* __restrict__ X, const int* __restrict__ perm,
-                                const double* __restrict__ sigma, c128* __restrict__ A, int rows,
-                                int chi_bond, int k, int mode) {
+__device__ __forceinline__ void gather_isometry_body(const c128* __restrict__ X, const int* __restrict__ perm,
+                                                     const double* __restrict__ sigma, c128* __restrict__ A,
+                                                     int rows, int chi_bond, int k, int mode, unsigned blk,
+                                                     unsigned nblk) {
   // rows = 2*chi_bond (length of each singular vector)
   const long total = (long)rows * k;
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
-       idx += (long)gridDim.x * blockDim.x) {
+  for (long idx = blk * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)nblk * blockDim.x) {
     const int kk = (int)(idx % k);
     const int r = (int)(idx / k);
     const double s = sigma[kk];
@@ -152,6 +196,17 @@ __global__ void gather_isometry(const c128* __restrict__ X, const int* __restric
       A[((long)p * k + kk) * chi_bond + a] = c128{v.re * inv, -v.im * inv};
     }
   }
+}
+__global__ void gather_isometry(const c128* __restrict__ X, const int* __restrict__ perm,
+                                const double* __restrict__ sigma, c128* __restrict__ A, int rows,
+                                int chi_bond, int k, int mode) {
+  gather_isometry_body(X, perm, sigma, A, rows, chi_bond, k, mode, blockIdx.x, gridDim.x);
+}
+__global__ void gather_isometry_grouped(const GRec<GatherIsoArgs>* __restrict__ recs, int n) {
+  unsigned bx, by, bz;
+  const GRec<GatherIsoArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
+  const GatherIsoArgs a = r.a;
+  gather_isometry_body(a.X, a.perm, a.sigma, a.A, a.rows, a.chi_bond, a.k, a.mode, bx, r.grid.x);
 }
 
 // Row-major (rows x cols) -> column-major copy: out[c*rows + r] = in[r*cols + c]
@@ -213,17 +268,22 @@ __global__ void qr_gather_right(const c128* __restrict__ Q, const c128* __restri
 
 // ------------------------------------------------------------------ lanczos
 constexpr int LZ_BLOCK = 256;
+constexpr int LZ_DIRECT_BLOCKS = 4096;   // in-kernel partial reductions up to this many blocks
 constexpr int LZ_EPT = 1;                     // vector elements per thread
 constexpr int LZ_ELEMS = LZ_BLOCK * LZ_EPT;   // elements per block in the partial reductions
 
-__global__ void lz_norm_partials(const c128* __restrict__ x, long n, double* __restrict__ part) {
+__device__ __forceinline__ void lz_norm_partials_body(const c128* __restrict__ x, long n,
+                                                      double* __restrict__ part, unsigned blk) {
   __shared__ double sh[16];
   double acc = 0;
-  const long base = (long)blockIdx.x * LZ_ELEMS;
+  const long base = (long)blk * LZ_ELEMS;
   for (long i = base + threadIdx.x; i < min(n, base + LZ_ELEMS); i += blockDim.x)
     acc += x[i].re * x[i].re + x[i].im * x[i].im;
   acc = block_sum(acc, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+  if (threadIdx.x == 0) part[blk] = acc;
+}
+__global__ void lz_norm_partials(const c128* __restrict__ x, long n, double* __restrict__ part) {
+  lz_norm_partials_body(x, n, part, blockIdx.x);
 }
 
 __global__ void lz_init_finalize(LanczosCtrl* ctrl, const double* __restrict__ part, int nblk) {
@@ -238,15 +298,20 @@ __global__ void lz_init_finalize(LanczosCtrl* ctrl, const double* __restrict__ p
   ctrl->hit_limit = 0;
 }
 
-__global__ void lz_scale_first(const c128* __restrict__ x, c128* __restrict__ v0, long n,
-                               const LanczosCtrl* ctrl, c64* __restrict__ v32) {
+__device__ __forceinline__ void lz_scale_first_body(const c128* __restrict__ x, c128* __restrict__ v0,
+                                                    long n, const LanczosCtrl* ctrl,
+                                                    c64* __restrict__ v32, unsigned blk, unsigned nblk) {
   const double b = ctrl->beta0;
   const double inv = b > 0 ? 1.0 / b : 0.0;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+  for (long i = blk * (long)blockDim.x + threadIdx.x; i < n; i += (long)nblk * blockDim.x) {
     const c128 v{x[i].re * inv, x[i].im * inv};
     v0[i] = v;
     if (v32) v32[i] = c64{float(v.re), float(v.im)};
   }
+}
+__global__ void lz_scale_first(const c128* __restrict__ x, c128* __restrict__ v0, long n,
+                               const LanczosCtrl* ctrl, c64* __restrict__ v32) {
+  lz_scale_first_body(x, v0, n, ctrl, v32, blockIdx.x, gridDim.x);
 }
 
 // This block's partial projections part[blk][i] = sum_e conj(V_i[e]) x[e],
@@ -291,12 +356,11 @@ __device__ void block_projections(const c128* __restrict__ V, long n, int j, lon
 // Partial projections c_i = <V_i, w> over this block's chunk.  Each thread
 // keeps its LZ_EPT elements of w in registers across the i loop.
 template <typename TW>
-__global__ __launch_bounds__(LZ_BLOCK) void lz_project(const LanczosCtrl* ctrl,
-                                                       const cplx<TW>* __restrict__ w,
-                                                       const c128* __restrict__ V, long n, int j,
-                                                       c128* __restrict__ part) {
+__device__ __forceinline__ void lz_project_body(const LanczosCtrl* ctrl, const cplx<TW>* __restrict__ w,
+                                                const c128* __restrict__ V, long n, int j,
+                                                c128* __restrict__ part, unsigned blk) {
   if (ctrl->done) return;
-  const long base = (long)blockIdx.x * LZ_ELEMS + threadIdx.x;
+  const long base = (long)blk * LZ_ELEMS + threadIdx.x;
   double wr[LZ_EPT], wi[LZ_EPT];
 #pragma unroll
   for (int r = 0; r < LZ_EPT; ++r) {
@@ -304,7 +368,14 @@ __global__ __launch_bounds__(LZ_BLOCK) void lz_project(const LanczosCtrl* ctrl,
     wr[r] = e < n ? double(w[e].re) : 0.0;
     wi[r] = e < n ? double(w[e].im) : 0.0;
   }
-  block_projections(V, n, j, base, wr, wi, part + (long)blockIdx.x * LZ_MAXK);
+  block_projections(V, n, j, base, wr, wi, part + (long)blk * LZ_MAXK);
+}
+template <typename TW>
+__global__ __launch_bounds__(LZ_BLOCK) void lz_project(const LanczosCtrl* ctrl,
+                                                       const cplx<TW>* __restrict__ w,
+                                                       const c128* __restrict__ V, long n, int j,
+                                                       c128* __restrict__ part) {
+  lz_project_body<TW>(ctrl, w, V, n, j, part, blockIdx.x);
 }
 
 // coef_i = sum over blocks of part_in[b][i] (i = 0..j), reduced by every block
@@ -314,17 +385,19 @@ __global__ __launch_bounds__(LZ_BLOCK) void lz_project(const LanczosCtrl* ctrl,
 // FP64), and the partial projections of wout (pass 1) or its partial squared
 // norm (pass 2) go to part_out, a different buffer from part_in.
 template <typename TW>
-__global__ __launch_bounds__(LZ_BLOCK) void lz_subtract(const LanczosCtrl* ctrl,
-                                                        const cplx<TW>* __restrict__ w,
-                                                        c128* __restrict__ wout,
-                                                        const c128* __restrict__ V, long n, int j,
-                                                        const c128* __restrict__ part_in, int nblk,
-                                                        c128* __restrict__ coef_out,
-                                                        c128* __restrict__ part_out, int pass) {
+__device__ __forceinline__ void lz_subtract_body(const LanczosCtrl* ctrl, const cplx<TW>* __restrict__ w,
+                                                 c128* __restrict__ wout, const c128* __restrict__ V,
+                                                 long n, int j, const c128* __restrict__ part_in,
+                                                 int nblk, c128* __restrict__ coef_out,
+                                                 c128* __restrict__ part_out, int pass, unsigned blk,
+                                                 const c128* __restrict__ coef_in = nullptr) {
   if (ctrl->done) return;
   __shared__ double sh[16];
   __shared__ c128 cs[LZ_MAXK];
-  {
+  if (coef_in) {
+    // coefficients reduced beforehand by lz_reduce (long vectors)
+    if (threadIdx.x <= j) cs[threadIdx.x] = coef_in[threadIdx.x];
+  } else {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int i = wv; i <= j; i += LZ_BLOCK / 64) {
       double ar = 0, ai = 0;
@@ -336,12 +409,12 @@ __global__ __launch_bounds__(LZ_BLOCK) void lz_subtract(const LanczosCtrl* ctrl,
       ai = wave_sum(ai);
       if (lane == 0) {
         cs[i] = c128{ar, ai};
-        if (blockIdx.x == 0) coef_out[i] = c128{ar, ai};
+        if (blk == 0) coef_out[i] = c128{ar, ai};
       }
     }
   }
   __syncthreads();
-  const long base = (long)blockIdx.x * LZ_ELEMS + threadIdx.x;
+  const long base = (long)blk * LZ_ELEMS + threadIdx.x;
   double xr[LZ_EPT], xi[LZ_EPT];
 #pragma unroll
   for (int r = 0; r < LZ_EPT; ++r) {
@@ -372,10 +445,46 @@ __global__ __launch_bounds__(LZ_BLOCK) void lz_subtract(const LanczosCtrl* ctrl,
   }
   if (pass == 2) {
     nrm = block_sum(nrm, sh);
-    if (threadIdx.x == 0) part_out[(long)blockIdx.x * LZ_MAXK] = c128{nrm, 0};
+    if (threadIdx.x == 0) part_out[(long)blk * LZ_MAXK] = c128{nrm, 0};
     return;
   }
-  block_projections(V, n, j, base, xr, xi, part_out + (long)blockIdx.x * LZ_MAXK);
+  block_projections(V, n, j, base, xr, xi, part_out + (long)blk * LZ_MAXK);
+}
+template <typename TW>
+__global__ __launch_bounds__(LZ_BLOCK) void lz_subtract(const LanczosCtrl* ctrl,
+                                                        const cplx<TW>* __restrict__ w,
+                                                        c128* __restrict__ wout,
+                                                        const c128* __restrict__ V, long n, int j,
+                                                        const c128* __restrict__ part_in, int nblk,
+                                                        c128* __restrict__ coef_out,
+                                                        c128* __restrict__ part_out, int pass,
+                                                        const c128* __restrict__ coef_in) {
+  lz_subtract_body<TW>(ctrl, w, wout, V, n, j, part_in, nblk, coef_out, part_out, pass, blockIdx.x,
+                       coef_in);
+}
+
+// Two-level reduction for long vectors: out[i] = sum_b part[b][i] for the
+// coefficients i = blockIdx.x (a fixed order: lanes stride over the blocks,
+// then the waves are combined in order).  With norm != 0, only the squared
+// norm partials part[b][0].re are summed, into ctrl->wnorm2.
+__global__ __launch_bounds__(256) void lz_reduce(LanczosCtrl* ctrl, const c128* __restrict__ part, int nblk,
+                                                 c128* __restrict__ out, int norm) {
+  if (ctrl->done) return;
+  __shared__ double sh[16];
+  const int i = blockIdx.x;
+  double ar = 0, ai = 0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    const c128 v = part[(long)b * LZ_MAXK + i];
+    ar += v.re;
+    ai += v.im;
+  }
+  ar = block_sum(ar, sh);
+  if (norm) {
+    if (threadIdx.x == 0) ctrl->wnorm2 = ar;
+    return;
+  }
+  ai = block_sum(ai, sh);
+  if (threadIdx.x == 0) out[i] = c128{ar, ai};
 }
 
 // beta_j = |w| from the per-block partial norms (one wave, fixed order).
@@ -452,6 +561,29 @@ __device__ void lz_close_iteration(LanczosCtrl* ctrl, double beta, const c128* _
 // 0 closes the iteration, and all blocks write V_{j+1} = w / beta_j, plus its
 // complex64 copy for the mixed path's next matvec when v32 is given.  Blocks
 // that start after convergence skip the write; V_{j+1} is not read then.
+__device__ __forceinline__ void lz_step_scale_body(LanczosCtrl* ctrl, const c128* __restrict__ part_norm,
+                                                   int nblk, const c128* __restrict__ coef1,
+                                                   const c128* __restrict__ coef2, int j, double tau,
+                                                   double tol, int max_m, const c128* __restrict__ w,
+                                                   c128* __restrict__ vnext, c64* __restrict__ v32,
+                                                   long n, unsigned blk, unsigned ngrid,
+                                                   int beta_reduced = 0) {
+  if (ctrl->done) return;
+  __shared__ double sbeta;
+  if (threadIdx.x < 64) {
+    const double b = beta_reduced ? sqrt(ctrl->wnorm2) : lz_beta(part_norm, nblk);
+    if (threadIdx.x == 0) sbeta = b;
+  }
+  __syncthreads();
+  const double beta = sbeta;
+  if (blk == 0 && threadIdx.x < 64) lz_close_iteration(ctrl, beta, coef1, coef2, j, tau, tol, max_m);
+  const double inv = 1.0 / beta;
+  for (long i = blk * (long)blockDim.x + threadIdx.x; i < n; i += (long)ngrid * blockDim.x) {
+    const c128 v{w[i].re * inv, w[i].im * inv};
+    vnext[i] = v;
+    if (v32) v32[i] = c64{float(v.re), float(v.im)};
+  }
+}
 __global__ __launch_bounds__(256) void lz_step_scale(LanczosCtrl* ctrl,
                                                      const c128* __restrict__ part_norm, int nblk,
                                                      const c128* __restrict__ coef1,
@@ -459,33 +591,21 @@ __global__ __launch_bounds__(256) void lz_step_scale(LanczosCtrl* ctrl,
                                                      double tau, double tol, int max_m,
                                                      const c128* __restrict__ w,
                                                      c128* __restrict__ vnext,
-                                                     c64* __restrict__ v32, long n) {
-  if (ctrl->done) return;
-  __shared__ double sbeta;
-  if (threadIdx.x < 64) {
-    const double b = lz_beta(part_norm, nblk);
-    if (threadIdx.x == 0) sbeta = b;
-  }
-  __syncthreads();
-  const double beta = sbeta;
-  if (blockIdx.x == 0 && threadIdx.x < 64) lz_close_iteration(ctrl, beta, coef1, coef2, j, tau, tol, max_m);
-  const double inv = 1.0 / beta;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const c128 v{w[i].re * inv, w[i].im * inv};
-    vnext[i] = v;
-    if (v32) v32[i] = c64{float(v.re), float(v.im)};
-  }
+                                                     c64* __restrict__ v32, long n, int beta_reduced) {
+  lz_step_scale_body(ctrl, part_norm, nblk, coef1, coef2, j, tau, tol, max_m, w, vnext, v32, n,
+                     blockIdx.x, gridDim.x, beta_reduced);
 }
 
 // y = beta0 * sum_{i<m} z_i V_i   (a zero input stays zero)
-__global__ void lz_combine(const LanczosCtrl* ctrl, const c128* __restrict__ V, long n,
-                           c128* __restrict__ y) {
+__device__ __forceinline__ void lz_combine_body(const LanczosCtrl* ctrl, const c128* __restrict__ V,
+                                                long n, c128* __restrict__ y, unsigned blk,
+                                                unsigned ngrid) {
   __shared__ c128 zs[LZ_MAXK];
   const int m = ctrl->m;
   const double b0 = ctrl->beta0;
   if (threadIdx.x < m) zs[threadIdx.x] = ctrl->z[threadIdx.x];
   __syncthreads();
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+  for (long e = blk * (long)blockDim.x + threadIdx.x; e < n; e += (long)ngrid * blockDim.x) {
     double r = 0, im = 0;
     for (int i = 0; i < m; ++i) {
       const c128 a = V[(long)i * n + e];
@@ -494,6 +614,69 @@ __global__ void lz_combine(const LanczosCtrl* ctrl, const c128* __restrict__ V, 
     }
     y[e] = c128{b0 * r, b0 * im};
   }
+}
+__global__ void lz_combine(const LanczosCtrl* ctrl, const c128* __restrict__ V, long n,
+                           c128* __restrict__ y) {
+  lz_combine_body(ctrl, V, n, y, blockIdx.x, gridDim.x);
+}
+
+// ---- grouped Lanczos: one record per instance (LzArgs), the iteration index
+// j shared by all instances of the launch (they iterate in lockstep; an
+// instance that has converged skips every kernel through its done flag).
+__global__ void lz_norm_partials_grouped(const GRec<LzArgs>* __restrict__ recs, int n) {
+  unsigned bx, by, bz;
+  const GRec<LzArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
+  lz_norm_partials_body(r.a.x, r.a.n, r.a.part_norm, bx);
+}
+__global__ void lz_init_finalize_grouped(const GRec<LzArgs>* __restrict__ recs, int n) {
+  if (threadIdx.x != 0) return;
+  const LzArgs& a = recs[blockIdx.x].a;
+  double sum = 0;
+  for (int b = 0; b < a.nblk; ++b) sum += a.part_norm[b];
+  LanczosCtrl* ctrl = a.ctrl;
+  ctrl->beta0 = sqrt(sum);
+  ctrl->done = (sum == 0.0) ? 1 : 0;
+  ctrl->m = 0;
+  ctrl->err = 0;
+  ctrl->zero = (sum == 0.0) ? 1 : 0;
+  ctrl->hit_limit = 0;
+}
+__global__ void lz_scale_first_grouped(const GRec<LzArgs>* __restrict__ recs, int n) {
+  unsigned bx, by, bz;
+  const GRec<LzArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
+  lz_scale_first_body(r.a.x, r.a.V, r.a.n, r.a.ctrl, r.a.v32, bx, r.grid.x);
+}
+template <typename TW>
+__global__ __launch_bounds__(LZ_BLOCK) void lz_project_grouped(const GRec<LzArgs>* __restrict__ recs, int n,
+                                                               int j) {
+  unsigned bx, by, bz;
+  const GRec<LzArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
+  lz_project_body<TW>(r.a.ctrl, reinterpret_cast<const cplx<TW>*>(r.a.w), r.a.V, r.a.n, j, r.a.part, bx);
+}
+template <typename TW>
+__global__ __launch_bounds__(LZ_BLOCK) void lz_subtract_grouped(const GRec<LzArgs>* __restrict__ recs, int n,
+                                                                int j, int pass) {
+  unsigned bx, by, bz;
+  const GRec<LzArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
+  const LzArgs a = r.a;
+  if (pass == 1)
+    lz_subtract_body<TW>(a.ctrl, reinterpret_cast<const cplx<TW>*>(a.w), a.w64, a.V, a.n, j, a.part,
+                         a.nblk, a.coef1, a.part2, 1, bx);
+  else
+    lz_subtract_body<double>(a.ctrl, a.w64, a.w64, a.V, a.n, j, a.part2, a.nblk, a.coef2, a.part, 2, bx);
+}
+__global__ __launch_bounds__(256) void lz_step_scale_grouped(const GRec<LzArgs>* __restrict__ recs, int n,
+                                                             int j) {
+  unsigned bx, by, bz;
+  const GRec<LzArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
+  const LzArgs a = r.a;
+  lz_step_scale_body(a.ctrl, a.part, a.nblk, a.coef1, a.coef2, j, a.tau, a.tol, a.max_m, a.w64,
+                     a.V + (long)(j + 1) * a.n, a.v32, a.n, bx, r.grid.x);
+}
+__global__ void lz_combine_grouped(const GRec<LzArgs>* __restrict__ recs, int n) {
+  unsigned bx, by, bz;
+  const GRec<LzArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
+  lz_combine_body(r.a.ctrl, r.a.V, r.a.n, r.a.y, bx, r.grid.x);
 }
 
 // ------------------------------------------------------------------ measurement
@@ -629,6 +812,146 @@ int grid_for(long n, int block = 256, int maxb = 8192) {
 
 }  // namespace
 
+// ------------------------------------------------------------------ word gather
+__global__ void gather_words_kernel(const void* const* __restrict__ src, int words, int n,
+                                    unsigned long long* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned long long* p = reinterpret_cast<const unsigned long long*>(src[i]);
+  for (int w = 0; w < words; ++w) out[(long)i * words + w] = p[w];
+}
+
+void launch_gather_words(const void* const* src, int words, int n, unsigned long long* out, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(gather_words_kernel, dim3(ceil_div(n, 64)), dim3(64), 0, s, src, words, n, out);
+  QCA_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ record arena
+void RecordArena::reserve(size_t bytes) {
+  if (bytes <= cap_) return;
+  QCA_CHECK(used_ == 0, "record arena grown while holding records");
+  release();
+  QCA_HIP(hipHostMalloc((void**)&host_, bytes));
+  QCA_HIP(hipMalloc((void**)&dev_, bytes));
+  cap_ = bytes;
+}
+
+void RecordArena::release() {
+  if (host_) (void)hipHostFree(host_);
+  if (dev_) (void)hipFree(dev_);
+  host_ = dev_ = nullptr;
+  cap_ = used_ = uploaded_ = 0;
+}
+
+const void* RecordArena::push_bytes(const void* src, size_t bytes) {
+  const size_t off = (used_ + 63) / 64 * 64;
+  QCA_CHECK(off + bytes <= cap_, "record arena full");
+  std::memcpy(host_ + off, src, bytes);
+  used_ = off + bytes;
+  return dev_ + off;
+}
+
+void RecordArena::upload(hipStream_t s) {
+  if (used_ > uploaded_) {
+    QCA_HIP(hipMemcpyAsync(dev_ + uploaded_, host_ + uploaded_, used_ - uploaded_, hipMemcpyHostToDevice, s));
+    uploaded_ = used_;
+  }
+}
+
+// ------------------------------------------------------------------ grouped launchers
+GRec<ConvArgs> conv_rec(const c128* in, c64* out, long n, const int* skip) {
+  return grec(ConvArgs{in, out, n, skip}, grid_for(n));
+}
+GRec<ThetaSvdArgs> theta_svd_rec(const c128* th, c128* X, int chil, int chir, int mode) {
+  return grec(ThetaSvdArgs{th, X, chil, chir, mode}, grid_for(4L * chil * chir));
+}
+GRec<GatherIsoArgs> gather_iso_rec(const c128* X, const int* perm, const double* sigma, c128* A,
+                                   int rows, int chi_bond, int k, int mode) {
+  return grec(GatherIsoArgs{X, perm, sigma, A, rows, chi_bond, k, mode}, grid_for((long)rows * k));
+}
+template <typename T>
+size_t wmix_lds_bytes(int nin, int nq, int Dout) {
+  return ((size_t)nin * WMIX_XC + (size_t)nq * WMIX_XC * Dout) * sizeof(cplx<T>);
+}
+template <typename T>
+GRec<WmixArgs<T>> wmix_rec(const cplx<T>* Tin, long in_stride, long nxc, int nin, cplx<T>* U, int nq,
+                           int Dout, const WSparseDev<T>& w, const int* skip) {
+  QCA_CHECK(Dout <= WMIX_DMAX, "MPO bond dimension above 16");
+  QCA_CHECK(nin <= 64, "too many input channels in MPO mixing");
+  QCA_CHECK(wmix_lds_bytes<T>(nin, nq, Dout) <= 150 * 1024, "MPO mixing stage exceeds LDS");
+  return grec(WmixArgs<T>{Tin, in_stride, nxc, nin, U, nq, Dout, w.rowptr, w.col, w.val, skip},
+              (unsigned)ceil_div(nxc, WMIX_XC));
+}
+template GRec<WmixArgs<double>> wmix_rec<double>(const c128*, long, long, int, c128*, int, int,
+                                                 const WSparseDev<double>&, const int*);
+template GRec<WmixArgs<float>> wmix_rec<float>(const c64*, long, long, int, c64*, int, int,
+                                               const WSparseDev<float>&, const int*);
+template size_t wmix_lds_bytes<double>(int, int, int);
+template size_t wmix_lds_bytes<float>(int, int, int);
+
+void launch_conv_grouped(const KGroup<ConvArgs>& g, hipStream_t s) {
+  if (g.empty()) return;
+  hipLaunchKernelGGL(convert_grouped, dim3(g.l.blocks), dim3(256), 0, s, g.l.recs, g.l.n);
+  QCA_HIP(hipGetLastError());
+}
+void launch_theta_svd_grouped(const KGroup<ThetaSvdArgs>& g, hipStream_t s) {
+  if (g.empty()) return;
+  hipLaunchKernelGGL(theta_to_svd_grouped, dim3(g.l.blocks), dim3(256), 0, s, g.l.recs, g.l.n);
+  QCA_HIP(hipGetLastError());
+}
+void launch_gather_iso_grouped(const KGroup<GatherIsoArgs>& g, hipStream_t s) {
+  if (g.empty()) return;
+  hipLaunchKernelGGL(gather_isometry_grouped, dim3(g.l.blocks), dim3(256), 0, s, g.l.recs, g.l.n);
+  QCA_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_wmix_grouped(const KGroup<WmixArgs<T>>& g, size_t lds, hipStream_t s) {
+  if (g.empty()) return;
+  ensure_dynamic_lds((const void*)wmix_grouped<T>, lds);
+  hipLaunchKernelGGL(wmix_grouped<T>, dim3(g.l.blocks), dim3(WMIX_XC * WMIX_WAVES), lds, s, g.l.recs, g.l.n);
+  QCA_HIP(hipGetLastError());
+}
+template void launch_wmix_grouped<double>(const KGroup<WmixArgs<double>>&, size_t, hipStream_t);
+template void launch_wmix_grouped<float>(const KGroup<WmixArgs<float>>&, size_t, hipStream_t);
+
+void LzGroup::prepare(const std::vector<LzArgs>& jobs, RecordArena& arena) {
+  std::vector<GRec<LzArgs>> a, b, c;
+  for (const LzArgs& j : jobs) {
+    QCA_CHECK(j.nblk == lanczos_nblk(j.n), "Lanczos block count does not match the vector");
+    a.push_back(grec(j, j.nblk));
+    b.push_back(grec(j, grid_for(j.n)));
+    c.push_back(grec(j, 1));
+  }
+  nb.prepare(a, arena);
+  gf.prepare(b, arena);
+  one.prepare(c, arena);
+}
+void LzGroup::begin(hipStream_t s) const {
+  if (empty()) return;
+  hipLaunchKernelGGL(lz_norm_partials_grouped, dim3(nb.l.blocks), dim3(LZ_BLOCK), 0, s, nb.l.recs, nb.l.n);
+  hipLaunchKernelGGL(lz_init_finalize_grouped, dim3(one.l.blocks), dim3(64), 0, s, one.l.recs, one.l.n);
+  hipLaunchKernelGGL(lz_scale_first_grouped, dim3(gf.l.blocks), dim3(256), 0, s, gf.l.recs, gf.l.n);
+  QCA_HIP(hipGetLastError());
+}
+template <typename TW>
+void LzGroup::tail(int j, hipStream_t s) const {
+  if (empty()) return;
+  QCA_CHECK(j + 1 < LZ_MAXK, "Krylov dimension limit");
+  hipLaunchKernelGGL(lz_project_grouped<TW>, dim3(nb.l.blocks), dim3(LZ_BLOCK), 0, s, nb.l.recs, nb.l.n, j);
+  hipLaunchKernelGGL(lz_subtract_grouped<TW>, dim3(nb.l.blocks), dim3(LZ_BLOCK), 0, s, nb.l.recs, nb.l.n, j, 1);
+  hipLaunchKernelGGL(lz_subtract_grouped<TW>, dim3(nb.l.blocks), dim3(LZ_BLOCK), 0, s, nb.l.recs, nb.l.n, j, 2);
+  hipLaunchKernelGGL(lz_step_scale_grouped, dim3(gf.l.blocks), dim3(256), 0, s, gf.l.recs, gf.l.n, j);
+  QCA_HIP(hipGetLastError());
+}
+template void LzGroup::tail<double>(int, hipStream_t) const;
+template void LzGroup::tail<float>(int, hipStream_t) const;
+void LzGroup::finish(hipStream_t s) const {
+  if (empty()) return;
+  hipLaunchKernelGGL(lz_combine_grouped, dim3(gf.l.blocks), dim3(256), 0, s, gf.l.recs, gf.l.n);
+  QCA_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ launchers
 template <typename T>
 void launch_wmix(const cplx<T>* Tin, long in_stride, long nxc, int nin, cplx<T>* U, int nq,
@@ -715,14 +1038,28 @@ void lanczos_iteration_tail(const cplx<TW>* w, long n, int j, double tau, double
   QCA_CHECK(j + 1 < lw.max_m && lw.max_m <= LZ_MAXK, "Krylov dimension limit");
   const int nb = lanczos_nblk(n);
   const LanczosCtrl* cc = lw.ctrl;
+  // Up to LZ_DIRECT_BLOCKS blocks every block reduces the previous partials
+  // itself (no extra launch); beyond that (state vectors of the exact path)
+  // that would be quadratic in the block count, so one launch per reduction
+  // sums them first.
+  const bool two_level = nb > LZ_DIRECT_BLOCKS;
   hipLaunchKernelGGL(lz_project<TW>, dim3(nb), dim3(LZ_BLOCK), 0, s, cc, w, lw.V, n, j, lw.part);
+  if (two_level)
+    hipLaunchKernelGGL(lz_reduce, dim3(j + 1), dim3(256), 0, s, lw.ctrl, (const c128*)lw.part, nb, lw.coef1, 0);
   hipLaunchKernelGGL(lz_subtract<TW>, dim3(nb), dim3(LZ_BLOCK), 0, s, cc, w, lw.w64, lw.V, n, j,
-                     (const c128*)lw.part, nb, lw.coef1, lw.part2, 1);
+                     (const c128*)lw.part, nb, lw.coef1, lw.part2, 1,
+                     two_level ? (const c128*)lw.coef1 : nullptr);
+  if (two_level)
+    hipLaunchKernelGGL(lz_reduce, dim3(j + 1), dim3(256), 0, s, lw.ctrl, (const c128*)lw.part2, nb, lw.coef2, 0);
   hipLaunchKernelGGL(lz_subtract<double>, dim3(nb), dim3(LZ_BLOCK), 0, s, cc, lw.w64, lw.w64, lw.V,
-                     n, j, (const c128*)lw.part2, nb, lw.coef2, lw.part, 2);
+                     n, j, (const c128*)lw.part2, nb, lw.coef2, lw.part, 2,
+                     two_level ? (const c128*)lw.coef2 : nullptr);
+  if (two_level)
+    hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, s, lw.ctrl, (const c128*)lw.part, nb, lw.coef2, 1);
   hipLaunchKernelGGL(lz_step_scale, dim3(grid_for(n)), dim3(256), 0, s, lw.ctrl,
                      (const c128*)lw.part, nb, (const c128*)lw.coef1, (const c128*)lw.coef2, j, tau,
-                     tol, lw.max_m, (const c128*)lw.w64, lw.V + (long)(j + 1) * n, v32, n);
+                     tol, lw.max_m, (const c128*)lw.w64, lw.V + (long)(j + 1) * n, v32, n,
+                     two_level ? 1 : 0);
   QCA_HIP(hipGetLastError());
 }
 template void lanczos_iteration_tail<double>(const c128*, long, int, double, double, LanczosWork&,

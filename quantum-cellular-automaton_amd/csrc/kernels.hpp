@@ -20,6 +20,7 @@ struct LanczosCtrl {
   double alpha[LZ_MAXK];
   double beta[LZ_MAXK];
   c128 z[LZ_MAXK];
+  double wnorm2;      // |w|^2 of the iteration (two-level reduction of long vectors)
 };
 
 // Workspace of one Lanczos exponential (vectors of length <= max_n).

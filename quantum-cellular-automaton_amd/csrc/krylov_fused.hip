@@ -22,7 +22,7 @@
 // by memory latency, so the launcher places the operands in LDS as far as
 // they fit (environments, sparse MPO, both intermediates, then leading basis
 // vectors) and the kernel reaches each one through a generic pointer.
-#include "kernels.hpp"
+#include "grouped.hpp"
 
 namespace qca {
 namespace {
@@ -365,9 +365,8 @@ __device__ double fk_tridiag_exp(const FkShared& sh, int j, double beta, double 
 }
 
 template <typename TE>
-__global__ __launch_bounds__(FK_THREADS) void fused_krylov_kernel(FusedKrylovArgs<TE> a) {
+__device__ __forceinline__ void fused_krylov_body(const FusedKrylovArgs<TE>& a, unsigned char* arena) {
   __shared__ FkShared sh;
-  extern __shared__ __align__(16) unsigned char arena[];
   const long n = a.n;
   const unsigned long long t0 = wall_clock64();
   FkCtx<TE> f;
@@ -500,10 +499,26 @@ __global__ __launch_bounds__(FK_THREADS) void fused_krylov_kernel(FusedKrylovArg
   }
 }
 
+template <typename TE>
+__global__ __launch_bounds__(FK_THREADS) void fused_krylov_kernel(FusedKrylovArgs<TE> a) {
+  extern __shared__ __align__(16) unsigned char arena[];
+  fused_krylov_body<TE>(a, arena);
+}
+
+// One exponential per workgroup, each with its own arguments and LDS placement
+// (the dynamic LDS of the launch is the largest placement).
+template <typename TE>
+__global__ __launch_bounds__(FK_THREADS) void fused_krylov_grouped(const GRec<FusedKrylovArgs<TE>>* __restrict__ recs,
+                                                                   int n) {
+  extern __shared__ __align__(16) unsigned char arena[];
+  const FusedKrylovArgs<TE> a = recs[blockIdx.x].a;
+  fused_krylov_body<TE>(a, arena);
+}
+
 }  // namespace
 
 template <typename TE>
-void launch_fused_krylov(const FusedKrylovArgs<TE>& a, hipStream_t s) {
+size_t fused_krylov_place(const FusedKrylovArgs<TE>& a, FusedKrylovArgs<TE>& b) {
   QCA_CHECK(a.n >= 1 && a.n <= FUSED_KRYLOV_MAX_N, "fused Krylov block above its size limit");
   QCA_CHECK(a.kind >= 0 && a.kind <= 2, "unknown effective-Hamiltonian kind");
   const int dq = a.kind == 0 ? 4 : (a.kind == 1 ? 2 : 1);
@@ -514,7 +529,7 @@ void launch_fused_krylov(const FusedKrylovArgs<TE>& a, hipStream_t s) {
   QCA_CHECK(a.kind == 2 || a.nnz >= 0, "negative MPO entry count");
   QCA_CHECK(a.kind == 2 || a.Dout <= 16, "MPO bond dimension above 16");
   // LDS placement, in order of use per Krylov iteration
-  FusedKrylovArgs<TE> b = a;
+  b = a;
   const long lr = (long)a.chil * a.chir;
   long off = a.n * (long)sizeof(c128);  // residual staging first: always in LDS
   b.offW = 0;
@@ -539,11 +554,30 @@ void launch_fused_krylov(const FusedKrylovArgs<TE>& a, hipStream_t s) {
   b.vcap = (int)std::min<long>(a.max_m - 1, (FK_LDS_BYTES - off) / vbytes);
   b.offV = b.vcap > 0 ? off : -1;
   if (b.vcap > 0) off += b.vcap * vbytes;
-  ensure_dynamic_lds((const void*)fused_krylov_kernel<TE>, (size_t)off);
-  hipLaunchKernelGGL(fused_krylov_kernel<TE>, dim3(1), dim3(FK_THREADS), (size_t)off, s, b);
+  return (size_t)off;
+}
+template size_t fused_krylov_place<double>(const FusedKrylovArgs<double>&, FusedKrylovArgs<double>&);
+template size_t fused_krylov_place<float>(const FusedKrylovArgs<float>&, FusedKrylovArgs<float>&);
+
+template <typename TE>
+void launch_fused_krylov(const FusedKrylovArgs<TE>& a, hipStream_t s) {
+  FusedKrylovArgs<TE> b;
+  const size_t off = fused_krylov_place(a, b);
+  ensure_dynamic_lds((const void*)fused_krylov_kernel<TE>, off);
+  hipLaunchKernelGGL(fused_krylov_kernel<TE>, dim3(1), dim3(FK_THREADS), off, s, b);
   QCA_HIP(hipGetLastError());
 }
 template void launch_fused_krylov<double>(const FusedKrylovArgs<double>&, hipStream_t);
 template void launch_fused_krylov<float>(const FusedKrylovArgs<float>&, hipStream_t);
+
+template <typename TE>
+void launch_fused_krylov_grouped(const KGroup<FusedKrylovArgs<TE>>& g, size_t lds, hipStream_t s) {
+  if (g.empty()) return;
+  ensure_dynamic_lds((const void*)fused_krylov_grouped<TE>, lds);
+  hipLaunchKernelGGL(fused_krylov_grouped<TE>, dim3(g.l.blocks), dim3(FK_THREADS), lds, s, g.l.recs, g.l.n);
+  QCA_HIP(hipGetLastError());
+}
+template void launch_fused_krylov_grouped<double>(const KGroup<FusedKrylovArgs<double>>&, size_t, hipStream_t);
+template void launch_fused_krylov_grouped<float>(const KGroup<FusedKrylovArgs<float>>&, size_t, hipStream_t);
 
 }  // namespace qca

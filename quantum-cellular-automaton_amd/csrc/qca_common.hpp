@@ -22,6 +22,10 @@ struct cplx {
 using c64 = cplx<float>;
 using c128 = cplx<double>;
 
+// Return code of a numerical method that stopped at its iteration limit
+// (QCA_ERR_NOT_CONVERGED in qca.h).
+constexpr int QCA_ERR_NOT_CONVERGED_CODE = -4;
+
 struct Error : std::runtime_error {
   int code;
   Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}

@@ -25,7 +25,7 @@
 // calls numpy.linalg.qr -> LAPACK), including LAPACK's zlarfg sign convention
 // beta = -sign(Re alpha)*|x|, so R has a real diagonal and Q matches LAPACK's
 // complete-QR complement columns.
-#include "kernels.hpp"
+#include "grouped.hpp"
 
 #include <algorithm>
 
@@ -79,8 +79,8 @@ __device__ void stage_tile(const c128* __restrict__ X, const JacobiGeom& g, int 
 }
 
 // ||X||_F^2 and reset of the control block (one workgroup).
-__global__ __launch_bounds__(JAC_NT) void jac_init(const c128* __restrict__ X, long n,
-                                                   JacobiCtrl* __restrict__ ctrl) {
+__device__ __forceinline__ void jac_init_body(const c128* __restrict__ X, long n,
+                                              JacobiCtrl* __restrict__ ctrl) {
   __shared__ double red[JAC_NT / 64];
   double part = 0;
   for (long e = threadIdx.x; e < n; e += JAC_NT) part += X[e].re * X[e].re + X[e].im * X[e].im;
@@ -96,15 +96,19 @@ __global__ __launch_bounds__(JAC_NT) void jac_init(const c128* __restrict__ X, l
   }
   if (threadIdx.x < JAC_MAX_SWEEPS) ctrl->rot[threadIdx.x] = 0;
 }
+__global__ __launch_bounds__(JAC_NT) void jac_init(const c128* __restrict__ X, long n,
+                                                   JacobiCtrl* __restrict__ ctrl) {
+  jac_init_body(X, n, ctrl);
+}
 
 // Partial Gram of pair blockIdx.x over row chunk blockIdx.y:
 // Gpart[p][kc](i, j) = sum_r conj(X(r, c_i)) X(r, c_j), stored column-major.
-__global__ __launch_bounds__(JAC_NT) void jac_gram(const c128* __restrict__ X, JacobiGeom g,
-                                                   int round, c128* __restrict__ Gpart,
-                                                   const JacobiCtrl* __restrict__ ctrl) {
+__device__ __forceinline__ void jac_gram_body(const c128* __restrict__ X, const JacobiGeom& g, int round,
+                                              c128* __restrict__ Gpart,
+                                              const JacobiCtrl* __restrict__ ctrl, int p, int kc) {
   if (ctrl->done) return;
   __shared__ c128 tile[JAC_RC * (JAC_LMAX + 1)];
-  const int p = blockIdx.x, kc = blockIdx.y, L = 2 * g.bs;
+  const int L = 2 * g.bs;
   const int r0 = kc * JAC_RC, nr = min(JAC_RC, g.rows - r0);
   stage_tile(X, g, p, round, r0, nr, tile);
   __syncthreads();
@@ -119,6 +123,11 @@ __global__ __launch_bounds__(JAC_NT) void jac_gram(const c128* __restrict__ X, J
     }
     out[e] = c128{re, im};
   }
+}
+__global__ __launch_bounds__(JAC_NT) void jac_gram(const c128* __restrict__ X, JacobiGeom g,
+                                                   int round, c128* __restrict__ Gpart,
+                                                   const JacobiCtrl* __restrict__ ctrl) {
+  jac_gram_body(X, g, round, Gpart, ctrl, blockIdx.x, blockIdx.y);
 }
 
 // One cyclic sweep of two-sided Jacobi on the Gram of pair blockIdx.x.
@@ -218,18 +227,16 @@ __device__ int inner_sweep(c128* G, c128* J, JRot (*rot)[JAC_LMAX / 2], int L, i
   return mine;
 }
 
-__global__ __launch_bounds__(JAC_NT) void jac_inner(JacobiGeom g, int sweep,
-                                                    const c128* __restrict__ Gpart,
-                                                    c128* __restrict__ Jout,
-                                                    int* __restrict__ rotated, double tol,
-                                                    JacobiCtrl* __restrict__ ctrl) {
+__device__ __forceinline__ void jac_inner_body(const JacobiGeom& g, int sweep, const c128* __restrict__ Gpart,
+                                               c128* __restrict__ Jout, int* __restrict__ rotated,
+                                               double tol, JacobiCtrl* __restrict__ ctrl, int p) {
   if (ctrl->done) return;
   constexpr int HM = JAC_LMAX / 2;
   __shared__ c128 G[JAC_LMAX * JAC_LMAX];
   __shared__ c128 J[JAC_LMAX * JAC_LMAX];
   __shared__ JRot rot[2][HM];
   __shared__ int total;
-  const int p = blockIdx.x, L = 2 * g.bs, H = g.bs, tid = threadIdx.x;
+  const int L = 2 * g.bs, H = g.bs, tid = threadIdx.x;
   const double negl = 1e-30 * ctrl->fro;
   for (int e = tid; e < L * L; e += JAC_NT) {
     double re = 0, im = 0;
@@ -253,23 +260,30 @@ __global__ __launch_bounds__(JAC_NT) void jac_inner(JacobiGeom g, int sweep,
     if (total) atomicAdd(&ctrl->rot[sweep], total);
   }
 }
+__global__ __launch_bounds__(JAC_NT) void jac_inner(JacobiGeom g, int sweep,
+                                                    const c128* __restrict__ Gpart,
+                                                    c128* __restrict__ Jout,
+                                                    int* __restrict__ rotated, double tol,
+                                                    JacobiCtrl* __restrict__ ctrl) {
+  jac_inner_body(g, sweep, Gpart, Jout, rotated, tol, ctrl, blockIdx.x);
+}
 
 // X_p <- X_p J_p on row chunk blockIdx.y of pair blockIdx.x; chunks from
 // g.kspl on belong to the second matrix Z (geometry gz), which receives the
 // same rotations (gz.kspl == 0 when there is none).
-__global__ __launch_bounds__(JAC_NT) void jac_apply(c128* __restrict__ Xa, JacobiGeom ga,
-                                                    c128* __restrict__ Za, JacobiGeom gz, int round,
-                                                    const c128* __restrict__ Jin,
-                                                    const int* __restrict__ rotated,
-                                                    const JacobiCtrl* __restrict__ ctrl) {
-  if (ctrl->done || rotated[blockIdx.x] == 0) return;
-  const bool second = (int)blockIdx.y >= ga.kspl;
+__device__ __forceinline__ void jac_apply_body(c128* __restrict__ Xa, const JacobiGeom& ga,
+                                               c128* __restrict__ Za, const JacobiGeom& gz, int round,
+                                               const c128* __restrict__ Jin,
+                                               const int* __restrict__ rotated,
+                                               const JacobiCtrl* __restrict__ ctrl, int p, int by) {
+  if (ctrl->done || rotated[p] == 0) return;
+  const bool second = by >= ga.kspl;
   c128* __restrict__ X = second ? Za : Xa;
   const JacobiGeom g = second ? gz : ga;
-  const int kc = second ? (int)blockIdx.y - ga.kspl : (int)blockIdx.y;
+  const int kc = second ? by - ga.kspl : by;
   __shared__ c128 tile[JAC_RC * (JAC_LMAX + 1)];
   __shared__ c128 J[JAC_LMAX * JAC_LMAX];
-  const int p = blockIdx.x, L = 2 * g.bs;
+  const int L = 2 * g.bs;
   const int r0 = kc * JAC_RC, nr = min(JAC_RC, g.rows - r0);
   stage_tile(X, g, p, round, r0, nr, tile);
   const c128* jp = Jin + (long)p * L * L;
@@ -289,6 +303,13 @@ __global__ __launch_bounds__(JAC_NT) void jac_apply(c128* __restrict__ Xa, Jacob
       }
       X[(long)gc * g.rows + r0 + r] = c128{re, im};
     }
+}
+__global__ __launch_bounds__(JAC_NT) void jac_apply(c128* __restrict__ Xa, JacobiGeom ga,
+                                                    c128* __restrict__ Za, JacobiGeom gz, int round,
+                                                    const c128* __restrict__ Jin,
+                                                    const int* __restrict__ rotated,
+                                                    const JacobiCtrl* __restrict__ ctrl) {
+  jac_apply_body(Xa, ga, Za, gz, round, Jin, rotated, ctrl, blockIdx.x, blockIdx.y);
 }
 
 // Closes a sweep: converged when it made no rotation.
@@ -380,11 +401,10 @@ __global__ __launch_bounds__(JAC_NT) void jac_finalize(const c128* __restrict__ 
 // sweeps of (Gram from the columns -> one rotation sweep on it -> X <- X J)
 // until a sweep makes no rotation, then finalize_body.  Same rotations, test
 // and accuracy as the kernel triple, without launches between rounds.
-__global__ __launch_bounds__(JAC_NT) void jac_small(c128* __restrict__ X, int rows, int cols,
-                                                    double* __restrict__ sigma,
-                                                    int* __restrict__ perm,
-                                                    SvdResult* __restrict__ res, int chi_max,
-                                                    double eps, int rank_cap, double tol) {
+__device__ __forceinline__ void jac_small_body(c128* __restrict__ X, int rows, int cols,
+                                               double* __restrict__ sigma, int* __restrict__ perm,
+                                               SvdResult* __restrict__ res, int chi_max, double eps,
+                                               int rank_cap, double tol) {
   constexpr int HM = JAC_LMAX / 2;
   __shared__ c128 tile[JAC_RC * (JAC_LMAX + 1)];
   __shared__ c128 G[JAC_LMAX * JAC_LMAX];
@@ -455,6 +475,13 @@ __global__ __launch_bounds__(JAC_NT) void jac_small(c128* __restrict__ X, int ro
   __syncthreads();
   finalize_body(X, rows, cols, sigma, perm, res, chi_max, eps, rank_cap,
                 converged ? sweep + 1 : sweep, converged);
+}
+__global__ __launch_bounds__(JAC_NT) void jac_small(c128* __restrict__ X, int rows, int cols,
+                                                    double* __restrict__ sigma,
+                                                    int* __restrict__ perm,
+                                                    SvdResult* __restrict__ res, int chi_max,
+                                                    double eps, int rank_cap, double tol) {
+  jac_small_body(X, rows, cols, sigma, perm, res, chi_max, eps, rank_cap, tol);
 }
 
 // ------------------------------------------------------------------ QR
@@ -585,10 +612,9 @@ __global__ __launch_bounds__(QR_THREADS) void householder_qr_kernel(c128* __rest
 constexpr int QRB_PB = 16;
 constexpr int QRB_MAXROWS = 512;
 
-__global__ __launch_bounds__(QR_THREADS) void qr_panel_kernel(c128* __restrict__ M, int rows, int j0,
-                                                              int pb, c128* __restrict__ tau,
-                                                              c128* __restrict__ V,
-                                                              c128* __restrict__ T) {
+__device__ __forceinline__ void qr_panel_body(c128* __restrict__ M, int rows, int j0, int pb,
+                                              c128* __restrict__ tau, c128* __restrict__ V,
+                                              c128* __restrict__ T) {
   __shared__ c128 P[QRB_PB][QRB_MAXROWS];   // panel, local row r = global row j0 + r
   __shared__ c128 s_tau[QRB_PB];
   __shared__ c128 s_vv[QRB_PB][QRB_PB];     // V^H V of the panel
@@ -706,19 +732,29 @@ __global__ __launch_bounds__(QR_THREADS) void qr_panel_kernel(c128* __restrict__
     }
   }
 }
+__global__ __launch_bounds__(QR_THREADS) void qr_panel_kernel(c128* __restrict__ M, int rows, int j0,
+                                                              int pb, c128* __restrict__ tau,
+                                                              c128* __restrict__ V,
+                                                              c128* __restrict__ T) {
+  qr_panel_body(M, rows, j0, pb, tau, V, T);
+}
 
-__global__ void set_identity(c128* __restrict__ Q, int rows, int cols) {
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)rows * cols;
-       e += (long)gridDim.x * blockDim.x) {
+__device__ __forceinline__ void set_identity_body(c128* __restrict__ Q, int rows, int cols, unsigned blk,
+                                                  unsigned nblk) {
+  for (long e = blk * (long)blockDim.x + threadIdx.x; e < (long)rows * cols;
+       e += (long)nblk * blockDim.x) {
     const int c = (int)(e / rows), r = (int)(e - (long)c * rows);
     Q[e] = c128{r == c ? 1.0 : 0.0, 0.0};
   }
 }
+__global__ void set_identity(c128* __restrict__ Q, int rows, int cols) {
+  set_identity_body(Q, rows, cols, blockIdx.x, gridDim.x);
+}
 
 // ---------------------------------------------------------- QR preconditioning
 // order[c]: columns of X (rows x cols) by descending norm (ties by index).
-__global__ __launch_bounds__(JAC_NT) void col_norm_order(const c128* __restrict__ X, int rows, int cols,
-                                                         int* __restrict__ order) {
+__device__ __forceinline__ void col_norm_order_body(const c128* __restrict__ X, int rows, int cols,
+                                                    int* __restrict__ order) {
   __shared__ double skey[JAC_MAXCOLS];
   __shared__ int sidx[JAC_MAXCOLS];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = JAC_NT / 64;
@@ -758,37 +794,50 @@ __global__ __launch_bounds__(JAC_NT) void col_norm_order(const c128* __restrict_
   }
   for (int e = tid; e < cols; e += JAC_NT) order[e] = sidx[e];
 }
+__global__ __launch_bounds__(JAC_NT) void col_norm_order(const c128* __restrict__ X, int rows, int cols,
+                                                         int* __restrict__ order) {
+  col_norm_order_body(X, rows, cols, order);
+}
 
 // Xp[:, c] = X[:, order[c]]
-__global__ void permute_cols(const c128* __restrict__ X, int rows, int cols,
-                             const int* __restrict__ order, c128* __restrict__ Xp) {
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)rows * cols;
-       e += (long)gridDim.x * blockDim.x) {
+__device__ __forceinline__ void permute_cols_body(const c128* __restrict__ X, int rows, int cols,
+                                                  const int* __restrict__ order, c128* __restrict__ Xp,
+                                                  unsigned blk, unsigned nblk) {
+  for (long e = blk * (long)blockDim.x + threadIdx.x; e < (long)rows * cols;
+       e += (long)nblk * blockDim.x) {
     const int c = (int)(e / rows), r = (int)(e - (long)c * rows);
     Xp[e] = X[(long)order[c] * rows + r];
   }
 }
+__global__ void permute_cols(const c128* __restrict__ X, int rows, int cols,
+                             const int* __restrict__ order, c128* __restrict__ Xp) {
+  permute_cols_body(X, rows, cols, order, Xp, blockIdx.x, gridDim.x);
+}
 
 // Y = R^H (cols x kq, column-major) from the R on/above the diagonal of the
 // QR-factored M (rows x cols): Y(c, k) = conj(R(k, c)), zero for k > c.
-__global__ void rh_from_qr(const c128* __restrict__ M, int rows, int cols, int kq,
-                           c128* __restrict__ Y) {
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)cols * kq;
-       e += (long)gridDim.x * blockDim.x) {
+__device__ __forceinline__ void rh_from_qr_body(const c128* __restrict__ M, int rows, int cols, int kq,
+                                                c128* __restrict__ Y, unsigned blk, unsigned nblk) {
+  for (long e = blk * (long)blockDim.x + threadIdx.x; e < (long)cols * kq;
+       e += (long)nblk * blockDim.x) {
     const int k = (int)(e / cols), c = (int)(e - (long)k * cols);
     const c128 v = (k <= c) ? M[(long)c * rows + k] : c128{0.0, 0.0};
     Y[e] = c128{v.re, -v.im};
   }
 }
+__global__ void rh_from_qr(const c128* __restrict__ M, int rows, int cols, int kq,
+                           c128* __restrict__ Y) {
+  rh_from_qr_body(M, rows, cols, kq, Y, blockIdx.x, gridDim.x);
+}
 
 // X[:, j] = |Y[:, j]| * Z[:, j], j < kq: the left singular vectors (columns of
 // Z = Q J, orthonormal) scaled by the singular values, the form X J = U S the
 // callers gather from.
-__global__ __launch_bounds__(JAC_NT) void scale_by_norms(const c128* __restrict__ Y, int yrows,
-                                                         const c128* __restrict__ Z, int rows, int kq,
-                                                         c128* __restrict__ X) {
+__device__ __forceinline__ void scale_by_norms_body(const c128* __restrict__ Y, int yrows,
+                                                    const c128* __restrict__ Z, int rows, int kq,
+                                                    c128* __restrict__ X, unsigned blk, unsigned nblk) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int j = blockIdx.x * (JAC_NT / 64) + wave; j < kq; j += gridDim.x * (JAC_NT / 64)) {
+  for (int j = blk * (JAC_NT / 64) + wave; j < kq; j += nblk * (JAC_NT / 64)) {
     double a = 0;
     for (int r = lane; r < yrows; r += 64) {
       const c128 v = Y[(long)j * yrows + r];
@@ -800,6 +849,144 @@ __global__ __launch_bounds__(JAC_NT) void scale_by_norms(const c128* __restrict_
       X[(long)j * rows + r] = c128{sg * z.re, sg * z.im};
     }
   }
+}
+__global__ __launch_bounds__(JAC_NT) void scale_by_norms(const c128* __restrict__ Y, int yrows,
+                                                         const c128* __restrict__ Z, int rows, int kq,
+                                                         c128* __restrict__ X) {
+  scale_by_norms_body(Y, yrows, Z, rows, kq, X, blockIdx.x, gridDim.x);
+}
+
+// ------------------------------------------------------------------ grouped kernels
+struct JacSmallArgs {
+  c128* X;
+  int rows, cols;
+  double* sigma;
+  int* perm;
+  SvdResult* res;
+  int chi_max;
+  double eps;
+  int rank_cap;
+  double tol;
+};
+struct ColOrderArgs {
+  const c128* X;
+  int rows, cols;
+  int* order;
+  c128* Xp;          // permute_cols output
+};
+struct RhArgs {
+  const c128* M;
+  int rows, cols, kq;
+  c128* Y;
+};
+struct ScaleArgs {
+  const c128* Y;
+  int yrows;
+  const c128* Z;
+  int rows, kq;
+  c128* X;
+};
+// The block Jacobi of one matrix (gram_jacobi's launch arguments).
+struct JacArgs {
+  c128* Y;
+  JacobiGeom g;
+  c128* Z;
+  JacobiGeom gz;
+  JacobiCtrl* ctrl;
+  int* rotated;
+  c128* Gpart;
+  c128* Jbuf;
+  double tol;
+};
+struct FinArgs {
+  const c128* X;
+  int rows, cols;
+  double* sigma;
+  int* perm;
+  SvdResult* res;
+  int chi_max;
+  double eps;
+  int rank_cap;
+  const JacobiCtrl* ctrl;
+};
+
+__global__ __launch_bounds__(JAC_NT) void jac_small_grouped(const GRec<JacSmallArgs>* __restrict__ recs, int n) {
+  const JacSmallArgs a = recs[blockIdx.x].a;
+  jac_small_body(a.X, a.rows, a.cols, a.sigma, a.perm, a.res, a.chi_max, a.eps, a.rank_cap, a.tol);
+}
+__global__ __launch_bounds__(JAC_NT) void col_norm_order_grouped(const GRec<ColOrderArgs>* __restrict__ recs,
+                                                                 int n) {
+  const ColOrderArgs a = recs[blockIdx.x].a;
+  col_norm_order_body(a.X, a.rows, a.cols, a.order);
+}
+__global__ void permute_cols_grouped(const GRec<ColOrderArgs>* __restrict__ recs, int n) {
+  unsigned bx, by, bz;
+  const GRec<ColOrderArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
+  permute_cols_body(r.a.X, r.a.rows, r.a.cols, r.a.order, r.a.Xp, bx, r.grid.x);
+}
+__global__ __launch_bounds__(QR_THREADS) void qr_panel_grouped(const GRec<QrPanelArgs>* __restrict__ recs, int n) {
+  const QrPanelArgs a = recs[blockIdx.x].a;
+  qr_panel_body(a.M, a.rows, a.j0, a.pb, a.tau, a.V, a.T);
+}
+__global__ void set_identity_grouped(const GRec<QrIdentArgs>* __restrict__ recs, int n) {
+  unsigned bx, by, bz;
+  const GRec<QrIdentArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
+  set_identity_body(r.a.Q, r.a.rows, r.a.cols, bx, r.grid.x);
+}
+__global__ void rh_from_qr_grouped(const GRec<RhArgs>* __restrict__ recs, int n) {
+  unsigned bx, by, bz;
+  const GRec<RhArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
+  rh_from_qr_body(r.a.M, r.a.rows, r.a.cols, r.a.kq, r.a.Y, bx, r.grid.x);
+}
+__global__ __launch_bounds__(JAC_NT) void scale_by_norms_grouped(const GRec<ScaleArgs>* __restrict__ recs, int n) {
+  unsigned bx, by, bz;
+  const GRec<ScaleArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
+  scale_by_norms_body(r.a.Y, r.a.yrows, r.a.Z, r.a.rows, r.a.kq, r.a.X, bx, r.grid.x);
+}
+__global__ __launch_bounds__(JAC_NT) void jac_init_grouped(const GRec<JacArgs>* __restrict__ recs, int n) {
+  const JacArgs& a = recs[blockIdx.x].a;
+  jac_init_body(a.Y, (long)a.g.rows * a.g.cols, a.ctrl);
+}
+// Rounds beyond a matrix's own tournament (nb - 1 rounds) are no-ops for it.
+__global__ __launch_bounds__(JAC_NT) void jac_gram_grouped(const GRec<JacArgs>* __restrict__ recs, int n,
+                                                           int round) {
+  unsigned bx, by, bz;
+  const GRec<JacArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
+  const JacobiGeom g = r.a.g;
+  if (round >= g.nb - 1) return;
+  jac_gram_body(r.a.Y, g, round, r.a.Gpart, r.a.ctrl, bx, by);
+}
+__global__ __launch_bounds__(JAC_NT) void jac_inner_grouped(const GRec<JacArgs>* __restrict__ recs, int n,
+                                                            int round, int sweep) {
+  unsigned bx, by, bz;
+  const GRec<JacArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
+  const JacobiGeom g = r.a.g;
+  if (round >= g.nb - 1) return;
+  jac_inner_body(g, sweep, r.a.Gpart, r.a.Jbuf, r.a.rotated, r.a.tol, r.a.ctrl, bx);
+}
+__global__ __launch_bounds__(JAC_NT) void jac_apply_grouped(const GRec<JacArgs>* __restrict__ recs, int n,
+                                                            int round) {
+  unsigned bx, by, bz;
+  const GRec<JacArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
+  const JacobiGeom g = r.a.g, gz = r.a.gz;
+  if (round >= g.nb - 1) return;
+  jac_apply_body(r.a.Y, g, r.a.Z, gz, round, r.a.Jbuf, r.a.rotated, r.a.ctrl, bx, by);
+}
+__global__ void jac_check_grouped(const GRec<JacArgs>* __restrict__ recs, int n, int sweep) {
+  JacobiCtrl* ctrl = recs[blockIdx.x].a.ctrl;
+  if (threadIdx.x != 0 || ctrl->done) return;
+  ctrl->sweeps = sweep + 1;
+  if (ctrl->rot[sweep] == 0) ctrl->done = 1;
+}
+// out[i] = done flag of matrix i (one thread per record)
+__global__ void jac_done_gather(const GRec<JacArgs>* __restrict__ recs, int n, int* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = recs[i].a.ctrl->done;
+}
+__global__ __launch_bounds__(JAC_NT) void jac_finalize_grouped(const GRec<FinArgs>* __restrict__ recs, int n) {
+  const FinArgs a = recs[blockIdx.x].a;
+  finalize_body(a.X, a.rows, a.cols, a.sigma, a.perm, a.res, a.chi_max, a.eps, a.rank_cap, a.ctrl->sweeps,
+                a.ctrl->done);
 }
 
 }  // namespace
@@ -936,8 +1123,12 @@ size_t qr_workspace_elems(int max_rows, int max_cols, int max_qcols) {
          + 32L * QRB_PB * w + 4096;                  // GEMM split-K partials
 }
 
-void launch_blocked_qr(c128* M, int rows, int cols, c128* tau, c128* Q, int qcols, c128* ws,
-                       size_t ws_elems, hipStream_t s) {
+// The blocked QR as a list of steps (panel factorisations, GEMMs, the
+// identity start of Q), each with a slot number that orders it among the steps
+// of any other matrix: the single-matrix launcher runs the steps in order, the
+// grouped one runs every slot for all matrices that have it.
+std::vector<QrStep> blocked_qr_steps(c128* M, int rows, int cols, c128* tau, c128* Q, int qcols,
+                                     c128* ws, size_t ws_elems) {
   QCA_CHECK(rows >= 1 && cols >= 1, "empty QR");
   QCA_CHECK(rows <= QRB_MAXROWS, "blocked QR rows above 512");
   const int kmax = std::min(rows, cols);
@@ -951,11 +1142,23 @@ void launch_blocked_qr(c128* M, int rows, int cols, c128* tau, c128* Q, int qcol
   const long used = gws - ws;
   QCA_CHECK((long)ws_elems > used + 1024, "QR workspace too small");
   const long gws_elems = (long)ws_elems - used;
+  std::vector<QrStep> out;
+  auto gemm_step = [&](int slot, const GemmDesc& d, const c128* A, const c128* B, c128* C) {
+    QrStep st{};
+    st.slot = slot;
+    st.kind = 1;
+    st.gemm = GemmJob<double>{d, A, B, C, gws, gws_elems};
+    out.push_back(st);
+  };
   // GEMM descriptors on column-major operands: X(r, c) at X + r + c * ld
   for (int p = 0; p < panels; ++p) {
     const int j0 = p * QRB_PB, pb = std::min(QRB_PB, kmax - j0);
     c128* Tp = T + (long)p * QRB_PB * QRB_PB;
-    hipLaunchKernelGGL(qr_panel_kernel, dim3(1), dim3(QR_THREADS), 0, s, M, rows, j0, pb, tau, V, Tp);
+    QrStep st{};
+    st.slot = QR_SLOT_FACTOR + 4 * p;
+    st.kind = 0;
+    st.panel = QrPanelArgs{M, rows, j0, pb, tau, V, Tp};
+    out.push_back(st);
     const int nt = cols - (j0 + pb), nr = rows - j0;
     if (nt <= 0) continue;
     const c128* Vp = V + (long)j0 * rows + j0;
@@ -966,27 +1169,32 @@ void launch_blocked_qr(c128* M, int rows, int cols, c128* tau, c128* Q, int qcol
     g.a_m = rows; g.a_k2 = 1; g.conj_a = 1;
     g.b_k2 = 1; g.b_n = rows;
     g.c_m = 1; g.c_n = pb;
-    gemm_c128(g, Vp, At, W, s, gws, gws_elems);
+    gemm_step(QR_SLOT_FACTOR + 4 * p + 1, g, Vp, At, W);
     // W2 = T^H W
     GemmDesc h{};
     h.M = pb; h.N = nt; h.K = pb; h.K2 = pb; h.NB1 = h.NB2 = 1; h.alpha_re = 1.0;
     h.a_m = QRB_PB; h.a_k2 = 1; h.conj_a = 1;
     h.b_k2 = 1; h.b_n = pb;
     h.c_m = 1; h.c_n = pb;
-    gemm_c128(h, Tp, W, W2, s, gws, gws_elems);
+    gemm_step(QR_SLOT_FACTOR + 4 * p + 2, h, Tp, W, W2);
     // A_trail -= V W2
     GemmDesc u{};
     u.M = nr; u.N = nt; u.K = pb; u.K2 = pb; u.NB1 = u.NB2 = 1; u.alpha_re = -1.0; u.accumulate = 1;
     u.a_m = 1; u.a_k2 = rows;
     u.b_k2 = 1; u.b_n = pb;
     u.c_m = 1; u.c_n = rows;
-    gemm_c128(u, Vp, W2, At, s, gws, gws_elems);
+    gemm_step(QR_SLOT_FACTOR + 4 * p + 3, u, Vp, W2, At);
   }
-  QCA_HIP(hipGetLastError());
-  if (Q == nullptr || qcols == 0) return;
+  if (Q == nullptr || qcols == 0) return out;
   // Q = H_0 ... H_{kk-1} [e_0 .. e_{qcols-1}], block reflectors applied from
   // the last panel back: Q(j0:, j0:) <- (I - V T V^H) Q(j0:, j0:)  (zungqr)
-  hipLaunchKernelGGL(set_identity, dim3(256), dim3(256), 0, s, Q, rows, qcols);
+  {
+    QrStep st{};
+    st.slot = QR_SLOT_IDENTITY;
+    st.kind = 2;
+    st.ident = QrIdentArgs{Q, rows, qcols};
+    out.push_back(st);
+  }
   const int kk = std::min(kmax, qcols);
   for (int p = (int)ceil_div(kk, QRB_PB) - 1; p >= 0; --p) {
     const int j0 = p * QRB_PB, pb = std::min(QRB_PB, kk - j0);
@@ -994,24 +1202,211 @@ void launch_blocked_qr(c128* M, int rows, int cols, c128* tau, c128* Q, int qcol
     const c128* Vp = V + (long)j0 * rows + j0;
     const c128* Tp = T + (long)p * QRB_PB * QRB_PB;
     c128* Qs = Q + (long)j0 * rows + j0;
+    const int base = QR_SLOT_Q + 3 * (QR_MAX_PANELS - 1 - p);
     GemmDesc g{};
     g.M = pb; g.N = nq; g.K = nr; g.K2 = nr; g.NB1 = g.NB2 = 1; g.alpha_re = 1.0;
     g.a_m = rows; g.a_k2 = 1; g.conj_a = 1;
     g.b_k2 = 1; g.b_n = rows;
     g.c_m = 1; g.c_n = pb;
-    gemm_c128(g, Vp, Qs, W, s, gws, gws_elems);
+    gemm_step(base, g, Vp, Qs, W);
     GemmDesc h{};
     h.M = pb; h.N = nq; h.K = pb; h.K2 = pb; h.NB1 = h.NB2 = 1; h.alpha_re = 1.0;
     h.a_m = 1; h.a_k2 = QRB_PB;
     h.b_k2 = 1; h.b_n = pb;
     h.c_m = 1; h.c_n = pb;
-    gemm_c128(h, Tp, W, W2, s, gws, gws_elems);
+    gemm_step(base + 1, h, Tp, W, W2);
     GemmDesc u{};
     u.M = nr; u.N = nq; u.K = pb; u.K2 = pb; u.NB1 = u.NB2 = 1; u.alpha_re = -1.0; u.accumulate = 1;
     u.a_m = 1; u.a_k2 = rows;
     u.b_k2 = 1; u.b_n = pb;
     u.c_m = 1; u.c_n = rows;
-    gemm_c128(u, Vp, W2, Qs, s, gws, gws_elems);
+    gemm_step(base + 2, u, Vp, W2, Qs);
+  }
+  return out;
+}
+
+void launch_blocked_qr(c128* M, int rows, int cols, c128* tau, c128* Q, int qcols, c128* ws,
+                       size_t ws_elems, hipStream_t s) {
+  for (const QrStep& st : blocked_qr_steps(M, rows, cols, tau, Q, qcols, ws, ws_elems)) {
+    if (st.kind == 0) {
+      const QrPanelArgs& a = st.panel;
+      hipLaunchKernelGGL(qr_panel_kernel, dim3(1), dim3(QR_THREADS), 0, s, a.M, a.rows, a.j0, a.pb, a.tau,
+                         a.V, a.T);
+    } else if (st.kind == 1) {
+      gemm_c128(st.gemm.d, st.gemm.A, st.gemm.B, st.gemm.C, s, st.gemm.ws, st.gemm.ws_elems);
+    } else {
+      hipLaunchKernelGGL(set_identity, dim3(256), dim3(256), 0, s, st.ident.Q, st.ident.rows, st.ident.cols);
+    }
+  }
+  QCA_HIP(hipGetLastError());
+}
+
+void grouped_jacobi_svd(const std::vector<SvdJob>& jobs, RecordArena& arena, GroupFlags& flags,
+                        hipStream_t s) {
+  std::vector<GRec<JacSmallArgs>> small;
+  std::vector<GRec<ColOrderArgs>> order1, perm;
+  std::vector<GRec<RhArgs>> rh;
+  std::vector<GRec<ScaleArgs>> scale;
+  std::vector<GRec<JacArgs>> jac_one, jac_gram_r, jac_inner_r, jac_apply_r;
+  std::vector<GRec<FinArgs>> fin;
+  std::vector<QrStep> steps;
+  int max_rounds = 0, pred = 2;
+  for (const SvdJob& j : jobs) {
+    QCA_CHECK(j.rows >= 1 && j.rows <= 512, "SVD rows out of range (1..512)");
+    QCA_CHECK(j.cols >= 1 && j.cols <= JAC_MAXCOLS, "SVD cols out of range (1..512)");
+    JacobiWork& w = *j.w;
+    QCA_CHECK(w.ws != nullptr, "Jacobi workspace not set");
+    if (j.cols <= JAC_LMAX) {
+      const double tol = w.tol > 0 ? w.tol : 1.1102230246251565e-16 * sqrt((double)j.rows);
+      small.push_back(grec(JacSmallArgs{j.X, j.rows, j.cols, j.sigma, j.perm, j.res, j.chi_max, j.eps,
+                                        j.rank_cap, tol}, 1));
+      continue;
+    }
+    const int kq = std::min(j.rows, j.cols);
+    c128 *Y = j.X, *Z = nullptr;
+    int yrows = j.rows, ycols = j.cols, zrows = 0;
+    const bool pre = w.pre != nullptr && kq > JAC_LMAX && w.pre_elems >= jacobi_precond_elems(j.rows, j.cols);
+    if (pre) {
+      c128* Xp = w.pre;
+      c128* Q = Xp + (long)j.rows * j.cols;
+      c128* Yp = Q + (long)j.rows * kq;
+      c128* tau = Yp + (long)j.cols * kq;
+      c128* qws = tau + kq + 64;
+      const size_t qws_elems = w.pre_elems - (size_t)(qws - w.pre);
+      QCA_CHECK(std::min(j.rows, j.cols) > 2 * QRB_PB && qws_elems >= qr_workspace_elems(j.rows, j.cols, kq),
+                "grouped SVD preconditioning needs the blocked QR");
+      const ColOrderArgs co{j.X, j.rows, j.cols, j.perm, Xp};
+      order1.push_back(grec(co, 1));
+      perm.push_back(grec(co, 256));
+      for (QrStep& st : blocked_qr_steps(Xp, j.rows, j.cols, tau, Q, kq, qws, qws_elems)) steps.push_back(st);
+      rh.push_back(grec(RhArgs{Xp, j.rows, j.cols, kq, Yp}, 256));
+      Y = Yp;
+      Z = Q;
+      yrows = j.cols;
+      ycols = kq;
+      zrows = j.rows;
+      scale.push_back(grec(ScaleArgs{Yp, j.cols, Q, j.rows, kq, j.X}, ceil_div(kq, JAC_NT / 64)));
+    }
+    const JacobiLayout l = jacobi_layout(yrows, ycols);
+    QCA_CHECK(l.bytes <= w.ws_bytes, "Jacobi workspace too small");
+    unsigned char* base = reinterpret_cast<unsigned char*>(w.ws);
+    JacArgs a{};
+    a.ctrl = reinterpret_cast<JacobiCtrl*>(base);
+    a.rotated = reinterpret_cast<int*>(base + 256);
+    a.Gpart = reinterpret_cast<c128*>(base + 256 + ((sizeof(int) * (l.nb / 2) + 15) / 16) * 16);
+    a.Jbuf = a.Gpart + l.gpart_elems;
+    a.Y = Y;
+    a.g = JacobiGeom{yrows, ycols, l.bs, l.nb, l.kspl};
+    a.Z = Z;
+    a.gz = JacobiGeom{zrows, ycols, l.bs, l.nb, Z ? ceil_div(zrows, JAC_RC) : 0};
+    a.tol = w.tol > 0 ? w.tol : 1.1102230246251565e-16 * sqrt((double)yrows);
+    const int np = l.nb / 2;
+    jac_one.push_back(grec(a, 1));
+    jac_gram_r.push_back(grec(a, np, l.kspl));
+    jac_inner_r.push_back(grec(a, np));
+    jac_apply_r.push_back(grec(a, np, l.kspl + a.gz.kspl));
+    fin.push_back(grec(FinArgs{Y, yrows, ycols, j.sigma, j.perm, j.res, j.chi_max, j.eps, j.rank_cap, a.ctrl}, 1));
+    max_rounds = std::max(max_rounds, l.nb - 1);
+    pred = std::max(pred, pre ? w.pred_sweeps_pre : w.pred_sweeps);
+  }
+  KGroup<JacSmallArgs> g_small;
+  KGroup<ColOrderArgs> g_order, g_perm;
+  KGroup<RhArgs> g_rh;
+  KGroup<ScaleArgs> g_scale;
+  KGroup<JacArgs> g_one, g_gram, g_inner, g_apply;
+  KGroup<FinArgs> g_fin;
+  g_small.prepare(small, arena);
+  g_order.prepare(order1, arena);
+  g_perm.prepare(perm, arena);
+  g_rh.prepare(rh, arena);
+  g_scale.prepare(scale, arena);
+  g_one.prepare(jac_one, arena);
+  g_gram.prepare(jac_gram_r, arena);
+  g_inner.prepare(jac_inner_r, arena);
+  g_apply.prepare(jac_apply_r, arena);
+  g_fin.prepare(fin, arena);
+  // blocked QR steps of every preconditioned matrix, slot by slot
+  std::sort(steps.begin(), steps.end(), [](const QrStep& x, const QrStep& y) { return x.slot < y.slot; });
+  struct Slot {
+    int kind;
+    KGroup<QrPanelArgs> panel;
+    KGroup<QrIdentArgs> ident;
+    GemmGroup<double> gemm;
+  };
+  std::vector<Slot> slots;
+  for (size_t i = 0; i < steps.size();) {
+    size_t e = i;
+    while (e < steps.size() && steps[e].slot == steps[i].slot) ++e;
+    Slot sl{};
+    sl.kind = steps[i].kind;
+    if (sl.kind == 0) {
+      std::vector<GRec<QrPanelArgs>> v;
+      for (size_t k = i; k < e; ++k) v.push_back(grec(steps[k].panel, 1));
+      sl.panel.prepare(v, arena);
+    } else if (sl.kind == 1) {
+      std::vector<GemmJob<double>> v;
+      for (size_t k = i; k < e; ++k) v.push_back(steps[k].gemm);
+      sl.gemm.prepare(v, arena);
+    } else {
+      std::vector<GRec<QrIdentArgs>> v;
+      for (size_t k = i; k < e; ++k) v.push_back(grec(steps[k].ident, 256));
+      sl.ident.prepare(v, arena);
+    }
+    slots.push_back(sl);
+    i = e;
+  }
+  QCA_CHECK((int)jac_one.size() <= flags.cap, "convergence flag buffer too small");
+  arena.upload(s);
+
+  if (!g_small.empty())
+    hipLaunchKernelGGL(jac_small_grouped, dim3(g_small.l.blocks), dim3(JAC_NT), 0, s, g_small.l.recs, g_small.l.n);
+  if (!g_order.empty()) {
+    hipLaunchKernelGGL(col_norm_order_grouped, dim3(g_order.l.blocks), dim3(JAC_NT), 0, s, g_order.l.recs,
+                       g_order.l.n);
+    hipLaunchKernelGGL(permute_cols_grouped, dim3(g_perm.l.blocks), dim3(256), 0, s, g_perm.l.recs, g_perm.l.n);
+  }
+  for (const Slot& sl : slots) {
+    if (sl.kind == 0)
+      hipLaunchKernelGGL(qr_panel_grouped, dim3(sl.panel.l.blocks), dim3(QR_THREADS), 0, s, sl.panel.l.recs,
+                         sl.panel.l.n);
+    else if (sl.kind == 1)
+      sl.gemm.launch(s);
+    else
+      hipLaunchKernelGGL(set_identity_grouped, dim3(sl.ident.l.blocks), dim3(256), 0, s, sl.ident.l.recs,
+                         sl.ident.l.n);
+  }
+  if (!g_rh.empty())
+    hipLaunchKernelGGL(rh_from_qr_grouped, dim3(g_rh.l.blocks), dim3(256), 0, s, g_rh.l.recs, g_rh.l.n);
+  QCA_HIP(hipGetLastError());
+  if (!g_one.empty()) {
+    hipLaunchKernelGGL(jac_init_grouped, dim3(g_one.l.blocks), dim3(JAC_NT), 0, s, g_one.l.recs, g_one.l.n);
+    const int nj = g_one.l.n;
+    int sweep = 0, chunk = std::max(2, pred);
+    while (sweep < JAC_MAX_SWEEPS) {
+      for (int c = 0; c < chunk && sweep < JAC_MAX_SWEEPS; ++c, ++sweep) {
+        for (int r = 0; r < max_rounds; ++r) {
+          hipLaunchKernelGGL(jac_gram_grouped, dim3(g_gram.l.blocks), dim3(JAC_NT), 0, s, g_gram.l.recs,
+                             g_gram.l.n, r);
+          hipLaunchKernelGGL(jac_inner_grouped, dim3(g_inner.l.blocks), dim3(JAC_NT), 0, s, g_inner.l.recs,
+                             g_inner.l.n, r, sweep);
+          hipLaunchKernelGGL(jac_apply_grouped, dim3(g_apply.l.blocks), dim3(JAC_NT), 0, s, g_apply.l.recs,
+                             g_apply.l.n, r);
+        }
+        hipLaunchKernelGGL(jac_check_grouped, dim3(nj), dim3(64), 0, s, g_one.l.recs, nj, sweep);
+      }
+      hipLaunchKernelGGL(jac_done_gather, dim3(ceil_div(nj, 256)), dim3(256), 0, s, g_one.l.recs, nj, flags.dev);
+      QCA_HIP(hipGetLastError());
+      QCA_HIP(hipMemcpyAsync(flags.host, flags.dev, sizeof(int) * nj, hipMemcpyDeviceToHost, s));
+      QCA_HIP(hipStreamSynchronize(s));
+      bool all = true;
+      for (int i = 0; i < nj; ++i) all = all && flags.host[i] != 0;
+      if (all) break;
+      chunk = 2;
+    }
+    hipLaunchKernelGGL(jac_finalize_grouped, dim3(g_fin.l.blocks), dim3(JAC_NT), 0, s, g_fin.l.recs, g_fin.l.n);
+    if (!g_scale.empty())
+      hipLaunchKernelGGL(scale_by_norms_grouped, dim3(g_scale.l.blocks), dim3(JAC_NT), 0, s, g_scale.l.recs,
+                         g_scale.l.n);
   }
   QCA_HIP(hipGetLastError());
 }

@@ -5,8 +5,10 @@ when a call fails; there is no host fallback.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -36,6 +38,11 @@ class Stats(ctypes.Structure):
                 ("svd_ms", c_double), ("krylov_ms", c_double)]
 
 
+class GroupStats(ctypes.Structure):
+    _fields_ = [("steps", c_long), ("krylov_iterations", c_long), ("host_syncs", c_long),
+                ("krylov_ms", c_double), ("svd_ms", c_double), ("total_ms", c_double)]
+
+
 # name -> (restype, argtypes); mirrors include/qca.h
 SIGNATURES = {
     "qca_version": (c_int, []),
@@ -44,6 +51,10 @@ SIGNATURES = {
     "qca_engine_create": (c_int, [P(EngineParams), P(c_void_p), P(c_int), P(c_int), P(c_void_p),
                                   P(c_int), P(c_void_p)]),
     "qca_engine_destroy": (c_int, [c_void_p]),
+    "qca_engine_footprint": (c_int, [P(EngineParams), P(c_void_p), P(c_int), P(c_int), P(c_int),
+                                     P(ctypes.c_size_t)]),
+    "qca_engine_device_bytes": (c_int, [c_void_p, P(ctypes.c_size_t)]),
+    "qca_device_memory": (c_int, [c_int, P(ctypes.c_size_t), P(ctypes.c_size_t)]),
     "qca_engine_step": (c_int, [c_void_p, c_int]),
     "qca_engine_measure": (c_int, [c_void_p, c_void_p]),
     "qca_engine_bond_dims": (c_int, [c_void_p, P(c_int)]),
@@ -54,6 +65,11 @@ SIGNATURES = {
     "qca_engine_observables": (c_int, [c_void_p, P(c_double), P(c_double), P(c_double), P(c_double)]),
     "qca_engine_gpu_ms": (c_int, [c_void_p, P(c_double)]),
     "qca_engine_set_fused_krylov_limit": (c_int, [c_void_p, c_long]),
+    "qca_group_create": (c_int, [P(c_void_p), c_int, P(c_void_p)]),
+    "qca_group_step": (c_int, [c_void_p, c_int]),
+    "qca_group_get_stats": (c_int, [c_void_p, P(GroupStats)]),
+    "qca_group_reset_stats": (c_int, [c_void_p]),
+    "qca_group_destroy": (c_int, [c_void_p]),
     "qca_heff_apply": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_int, c_int, c_int, c_int, c_void_p]),
     "qca_env_update": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
@@ -71,6 +87,32 @@ SIGNATURES = {
 }
 
 _lib = None
+# Engines still open; closed explicitly at interpreter exit (below).
+_live_engines: "weakref.WeakSet" = weakref.WeakSet()
+_live_groups: "weakref.WeakSet" = weakref.WeakSet()
+
+
+def shutdown() -> None:
+    """Close every open engine and release the primitives' device workspace
+    (qca_engine_destroy, qca_release_workspace).  Registered with ``atexit``,
+    so it runs while the HIP runtime is still up: device memory, streams and
+    events are never left to ``__del__`` during interpreter finalisation or to
+    process-exit teardown."""
+    for g in list(_live_groups):
+        try:
+            g.close()
+        except Exception:
+            pass
+    for e in list(_live_engines):
+        try:
+            e.close()
+        except Exception:
+            pass
+    if _lib is not None:
+        _lib.qca_release_workspace()
+
+
+atexit.register(shutdown)
 
 
 def lib():
@@ -120,6 +162,32 @@ def device_count() -> int:
     return n.value
 
 
+def device_memory(device: int = 0) -> tuple:
+    """(free, total) bytes of a HIP device (qca_device_memory)."""
+    f, t = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    check(lib().qca_device_memory(device, ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value
+
+
+def engine_footprint(Ws, chis, *, algorithm: int, precision: int, chi_max: int) -> int:
+    """Device bytes an Engine of these MPO tensors, initial bond dimensions
+    (N+1 values) and parameters would allocate (qca_engine_footprint; host
+    only, no device needed)."""
+    n = len(Ws)
+    Wc = [np.ascontiguousarray(W, dtype=np.complex128) for W in Ws]
+    if len(chis) != n + 1:
+        raise ValueError("N+1 bond dimensions are required")
+    prm = EngineParams(device=0, ncells=n, algorithm=algorithm, precision=precision, chi_max=chi_max,
+                       reserved=0, dt=0.1, eps=1e-10, krylov_tol=0.0)
+    Wp = (c_void_p * n)(*[W.ctypes.data for W in Wc])
+    dl = (c_int * n)(*[W.shape[2] for W in Wc])
+    dr = (c_int * n)(*[W.shape[3] for W in Wc])
+    ch = (c_int * (n + 1))(*[int(c) for c in chis])
+    out = ctypes.c_size_t(0)
+    check(lib().qca_engine_footprint(ctypes.byref(prm), Wp, dl, dr, ch, ctypes.byref(out)))
+    return out.value
+
+
 class Engine:
     """Owning handle of a device-resident TDVP instance (qca_engine_*)."""
 
@@ -146,8 +214,15 @@ class Engine:
         h = c_void_p()
         check(lib().qca_engine_create(ctypes.byref(prm), Wp, dl, dr, Ap, ch, ctypes.byref(h)))
         self._h = h
+        _live_engines.add(self)
         if fused_krylov_max_n is not None:
             self.set_fused_krylov_limit(fused_krylov_max_n)
+
+    def device_bytes(self) -> int:
+        """Device bytes held by the engine (qca_engine_device_bytes)."""
+        out = ctypes.c_size_t(0)
+        check(lib().qca_engine_device_bytes(self._h, ctypes.byref(out)))
+        return out.value
 
     def set_fused_krylov_limit(self, max_n: int) -> None:
         """Krylov blocks of at most ``max_n`` elements use the one-launch
@@ -231,3 +306,43 @@ def batch_step(engines, nsteps: int = 1, nthreads: int = 8) -> None:
     failed = c_int(-1)
     rc = lib().qca_batch_step(hs, len(engines), nsteps, nthreads, ctypes.byref(failed))
     check(rc)
+
+
+class Group:
+    """Lockstep grouped stepping of several 2TDVP engines (qca_group_*): one
+    launch per kernel per sweep position for all of them.  Keeps references
+    to its engines so that they outlive it."""
+
+    def __init__(self, engines):
+        self.engines = list(engines)
+        if not self.engines:
+            raise ValueError("a group needs at least one engine")
+        hs = (c_void_p * len(self.engines))(*[e._h for e in self.engines])
+        h = c_void_p()
+        check(lib().qca_group_create(hs, len(self.engines), ctypes.byref(h)))
+        self._h = h
+        _live_groups.add(self)
+
+    def step(self, nsteps: int = 1) -> None:
+        """nsteps time steps of every engine (qca_group_step)."""
+        check(lib().qca_group_step(self._h, nsteps))
+
+    def stats(self) -> dict:
+        s = GroupStats()
+        check(lib().qca_group_get_stats(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in GroupStats._fields_}
+
+    def reset_stats(self) -> None:
+        check(lib().qca_group_reset_stats(self._h))
+
+    def close(self) -> None:
+        """Release the group (qca_group_destroy); the engines stay open."""
+        if getattr(self, "_h", None):
+            check(lib().qca_group_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

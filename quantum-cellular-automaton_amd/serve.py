@@ -11,9 +11,12 @@ threads repeatedly take the front request, advance it by one reporting chunk
 on its own engine (its own HIP stream), measure, and requeue it until done.
 New arrivals join the queue while others run (continuous batching), so a short
 request is never stuck behind a long one.  A request is admitted when its
-estimated device footprint fits in the remaining budget; otherwise it waits in
-arrival order.  The engine factory is injectable, so the scheduler is tested
-on the host with a stand-in engine.
+device footprint fits in the remaining budget; otherwise it waits in arrival
+order.  With the GPU engine factory the footprint is the exact number of
+bytes the engine will allocate (qca_engine_footprint, the constructor's own
+sizing) and the budget is the device's free memory at start-up, less a
+reserve.  The engine factory is injectable, so the scheduler is tested on the
+host with a stand-in engine.
 """
 from __future__ import annotations
 
@@ -28,8 +31,9 @@ import workloads
 
 
 def footprint_bytes(inst: workloads.Instance, precision: int = 1) -> int:
-    """Upper bound of an engine's device memory: MPS (c128), environments
-    (c64 mixed / c128 plain), Krylov basis (64 vectors of the largest 2-site
+    """Closed-form upper bound of an engine's device memory, for factories
+    that do not report their own footprint: MPS (c128), environments (c64
+    mixed / c128 plain), Krylov basis (64 vectors of the largest 2-site
     block) and per-engine workspaces."""
     n, chi = inst.ncells, inst.chi_max
     dims = [min(2 ** min(i, 40), 2 ** min(n - i, 40), chi) for i in range(n + 1)]
@@ -56,14 +60,28 @@ class Request:
     footprint: int = 0
 
 
+# Device memory kept free for the runtime, the primitives' workspace and
+# allocation granularity when the budget comes from the free memory.
+RESERVE_BYTES = 4 << 30
+
+
 class Server:
-    def __init__(self, make_engine, workers: int = 8, budget_bytes: int = 200 << 30,
-                 report_every: int = 10, precision: int = 1):
+    def __init__(self, make_engine, workers: int = 8, budget_bytes: int | None = None,
+                 report_every: int = 10, precision: int = 1, device: int = 0):
+        """``make_engine(inst)`` builds an engine; when it has a
+        ``footprint(inst)`` attribute (the GPU factory), that is the admission
+        size, else the closed-form bound ``footprint_bytes``.  ``budget_bytes``
+        defaults to the device's free memory less RESERVE_BYTES."""
         self.make_engine = make_engine
         self.workers = workers
+        if budget_bytes is None:
+            import qca_native as qn
+            free, _ = qn.device_memory(device)
+            budget_bytes = max(0, free - RESERVE_BYTES)
         self.budget = budget_bytes
         self.report_every = report_every
         self.precision = precision
+        self._footprint = getattr(make_engine, "footprint", None)
         self._cv = threading.Condition()
         self._pending = collections.deque()   # arrived, not admitted (arrival order)
         self._ready = collections.deque()     # admitted, waiting for a worker
@@ -76,7 +94,8 @@ class Server:
 
     # ------------------------------------------------------------ intake
     def submit(self, req: Request) -> None:
-        req.footprint = footprint_bytes(req.inst, self.precision)
+        req.footprint = (self._footprint(req.inst) if self._footprint is not None
+                         else footprint_bytes(req.inst, self.precision))
         if req.footprint > self.budget:
             raise ValueError(f"request {req.rid} needs {req.footprint} B, above the budget")
         with self._cv:
@@ -179,17 +198,35 @@ def poisson_arrivals(rate: float, window_s: float, seed: int = 5, big_fraction: 
         rid += 1
 
 
-def gpu_engine_factory(precision: int = 1, chi_cap: int | None = None,
-                       fused_krylov_max_n: int = 4096):
+class GpuEngineFactory:
     """Engine factory for the GPU path: one qca_engine per request.  Requests
     share the GPU, so Krylov blocks up to ``fused_krylov_max_n`` elements use
-    the one-launch exponential (throughput over single-request latency)."""
-    import qca_native as qn
+    the one-launch exponential (throughput over single-request latency).
+    ``footprint(inst)`` is the exact device size of the engine ``self(inst)``
+    builds (qca_engine_footprint)."""
 
-    def make(inst: workloads.Instance):
-        chi = inst.chi_max if chi_cap is None else min(inst.chi_max, chi_cap)
+    def __init__(self, precision: int = 1, chi_cap: int | None = None, fused_krylov_max_n: int = 4096):
+        self.precision = precision
+        self.chi_cap = chi_cap
+        self.fused_krylov_max_n = fused_krylov_max_n
+
+    def _chi(self, inst: workloads.Instance) -> int:
+        return inst.chi_max if self.chi_cap is None else min(inst.chi_max, self.chi_cap)
+
+    def footprint(self, inst: workloads.Instance) -> int:
+        import qca_native as qn
+        return qn.engine_footprint(workloads.mpo_tensors(inst), [1] * (inst.ncells + 1), algorithm=2,
+                                   precision=self.precision, chi_max=self._chi(inst))
+
+    def __call__(self, inst: workloads.Instance):
+        import qca_native as qn
         return qn.Engine(workloads.mpo_tensors(inst), workloads.initial_tensors(inst), algorithm=2,
-                         precision=precision, chi_max=chi, dt=inst.step_size, eps=inst.svd_epsilon,
-                         krylov_tol=1e-10 if precision == 1 else 1e-14,
-                         fused_krylov_max_n=fused_krylov_max_n)
-    return make
+                         precision=self.precision, chi_max=self._chi(inst), dt=inst.step_size,
+                         eps=inst.svd_epsilon, krylov_tol=1e-10 if self.precision == 1 else 1e-14,
+                         fused_krylov_max_n=self.fused_krylov_max_n)
+
+
+def gpu_engine_factory(precision: int = 1, chi_cap: int | None = None,
+                       fused_krylov_max_n: int = 4096) -> GpuEngineFactory:
+    """The GPU engine factory (see GpuEngineFactory)."""
+    return GpuEngineFactory(precision, chi_cap, fused_krylov_max_n)

@@ -327,8 +327,39 @@ def g_a1tdvp():
     _save("a1tdvp_runs", **out)
 
 
+def g_states():
+    """Occupation probabilities of the reference's named initial states
+    (states.py:11-88) for a few (ncells, distance) pairs, read back from the
+    product MPS each function builds (A[i][:, 0, 0] = (sqrt(1-p), sqrt(p))).
+
+    The reference's functions take ncells and distance from the parsed
+    arguments at call time, except ``single``, whose default position is
+    bound at import time to the command line's ncells (states.py:34); it is
+    called here with that binding, ``single(int(ncells / 2))``, as a run with
+    ``--num-cells ncells`` sees it.  ``rand`` draws from the unseeded
+    ``random`` module (states.py:86-88) and is recorded for its support only."""
+    import states as ref_states
+    names = ["blinker", "triple_blinker", "full_blinker", "single", "single_bottom", "all_ket_0",
+             "all_ket_1", "only_outer", "all_ket_1_but_outer", "equal_superposition",
+             "equal_superposition_but_outer", "gradient", "gradient_reverse", "rand"]
+    out = {"names": np.array(names)}
+    cases = [(9, 1), (12, 2), (15, 1), (41, 2)]
+    out["cases"] = np.array(cases)
+    for c, (n, d) in enumerate(cases):
+        ARGS.rules = Rules(ncells=n, activation_interval=range(1, 2), distance=d, periodic=False)
+        for name in names:
+            if name == "single":
+                mps = ref_states.single(int(n / 2))
+            elif name == "gradient_reverse":
+                mps = ref_states.gradient(reverse=True)
+            else:
+                mps = getattr(ref_states, name)()
+            out[f"c{c}_{name}"] = np.array([abs(A[1, 0, 0]) ** 2 for A in mps.A])
+    _save("named_states", **out)
+
+
 ALL = dict(padded=g_padded, mpo=g_mpo, exact=g_exact, fullchi=g_fullchi, units=g_units, short=g_short,
-           config1=g_config1, a1tdvp=g_a1tdvp)
+           config1=g_config1, a1tdvp=g_a1tdvp, states=g_states)
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()

@@ -182,7 +182,7 @@ def test_server_rejects_oversized_and_reports_errors():
         def __init__(self, inst):
             raise RuntimeError("device lost")
     small = workloads.batch_instances(2, seed=4)
-    srv = serve.Server(Boom, workers=2)
+    srv = serve.Server(Boom, workers=2, budget_bytes=200 << 30)
     with pytest.raises(RuntimeError, match="device lost"):
         srv.run([(0.0, serve.Request(i, s, 20, 0.0)) for i, s in enumerate(small)])
 
@@ -245,3 +245,51 @@ def test_output_file_names_follow_the_reference():
     assert g.file_name("single") == "single15_112_01_a1tdvp_16_taylor_5"
     g.args = args("a1tdvp", "exact_exponential")
     assert g.file_name("0") == "015_112_01_a1tdvp_16_exact_exponential"
+
+
+def test_named_states_match_reference_golden():
+    """Every named initial state (states.NAMED, reference states.py:11-88)
+    sets the same occupation probabilities as the reference's function of the
+    same name, for several (ncells, distance) pairs; ``rand`` (unseeded in the
+    reference) is checked for its support and its seeded reproducibility."""
+    import states
+    g = golden("named_states")
+    names = [str(x) for x in g["names"]]
+    for c, (n, d) in enumerate(g["cases"]):
+        n, d = int(n), int(d)
+        for name in names:
+            ref = g[f"c{c}_{name}"]
+            if name == "gradient_reverse":
+                mps = states.gradient(n, reverse=True)
+            else:
+                mps = states.NAMED[name](n, d)
+            got = np.array([abs(A[1, 0, 0]) ** 2 for A in mps.A])
+            assert mps.A[0].shape == (2, 1, 1) and len(mps.A) == n
+            if name == "rand":
+                assert set(np.unique(got)) <= {0.0, 1.0} and set(np.unique(ref)) <= {0.0, 1.0}
+                again = np.array([abs(A[1, 0, 0]) ** 2 for A in states.NAMED[name](n, d).A])
+                assert np.array_equal(got, again)
+                continue
+            assert np.abs(got - ref).max() < 1e-15, (n, d, name)
+
+
+def test_engine_footprint_is_host_computed_and_ordered():
+    """qca_engine_footprint sizes an engine on the host (no device): it grows
+    with the bond cap and the chain, mixed environments are smaller than
+    complex128 ones, and the GPU serving factory admits by it."""
+    import qca_native as qn
+    import serve
+    import workloads
+    inst = workloads.batch_instances(1, seed=4)[0]
+    Ws = workloads.mpo_tensors(inst)
+    ch = [1] * (inst.ncells + 1)
+    f = {(chi, prec): qn.engine_footprint(Ws, ch, algorithm=2, precision=prec, chi_max=chi)
+         for chi in (16, 64, 128) for prec in (0, 1)}
+    assert f[(16, 1)] < f[(64, 1)] < f[(128, 1)]
+    assert f[(128, 1)] < f[(128, 0)]
+    fac = serve.gpu_engine_factory(precision=1, chi_cap=64)
+    assert fac.footprint(inst) == f[(min(inst.chi_max, 64), 1)]
+    big = workloads.Instance(101, 256, 2, range(2, 4), [0.0] * 101)
+    assert fac.footprint(big) < serve.gpu_engine_factory(precision=1).footprint(big)
+    with pytest.raises(qn.QcaError, match="max bond"):
+        qn.engine_footprint(Ws, ch, algorithm=2, precision=1, chi_max=300)
