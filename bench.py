@@ -201,6 +201,10 @@ def batch128_bench(torch, a, tensors, fp64_rate):
            "instances": B, "warmup_steps": a.batch128_warmup, "timed_steps": a.batch128_steps,
            "fp64_single_site_updates_per_s": fp64_rate}
     for method in ("grouped", "streams"):
+        if method == "streams":
+            B = a.batch128_streams
+            if not B:
+                continue
         log(f"batched chi=128 ({method}): {B} instances of the prepared state")
         engines = [qn.Engine(Ws, tensors, algorithm=2, precision=1, chi_max=CHI, dt=DT, eps=EPS,
                              krylov_tol=a.mixed_krylov_tol, fused_krylov_max_n=4096) for _ in range(B)]
@@ -222,7 +226,7 @@ def batch128_bench(torch, a, tensors, fp64_rate):
         su = sum(e.stats()["site_updates"] for e in engines)
         bd1 = engines[0].bond_dims()
         rate = su / dt
-        res = {"site_updates": su, "seconds": dt, "site_updates_per_s": rate,
+        res = {"instances": B, "site_updates": su, "seconds": dt, "site_updates_per_s": rate,
                "speedup_over_fp64_single": rate / fp64_rate,
                "bonds_at_chi_max": {"start": int((bd0 == CHI).sum()), "end": int((bd1 == CHI).sum())}}
         if grp is not None:
@@ -313,17 +317,24 @@ def serve_bench(a):
     `a.serve_window` s (default_rng(5)), each request a batched-workload draw
     (or, with probability `a.serve_big_fraction`, the 101-cell chi-256 case),
     steps capped at `a.serve_max_steps` and chi at `a.serve_chi_cap`; <n_i>
-    returned every 10 steps; `a.batch_threads` workers; mixed precision."""
+    returned every 10 steps; mixed precision.  Admission by each engine's exact
+    device footprint against the device's free memory; every admitted request
+    advances one step per tick in one grouped step (serve.GroupServer), or with
+    --serve-method streams on its own stream with `a.batch_threads` workers."""
     import serve
     arrivals = serve.poisson_arrivals(a.serve, a.serve_window, seed=5, big_fraction=a.serve_big_fraction,
                                       max_steps=a.serve_max_steps)
-    log(f"served window: {len(arrivals)} requests over {a.serve_window:.0f} s")
+    log(f"served window ({a.serve_method}): {len(arrivals)} requests over {a.serve_window:.0f} s")
     cap = a.serve_chi_cap or None
-    srv = serve.Server(serve.gpu_engine_factory(precision=1, chi_cap=cap), workers=a.batch_threads)
+    factory = serve.gpu_engine_factory(precision=1, chi_cap=cap)
+    if a.serve_method == "group":
+        srv = serve.GroupServer(factory)
+    else:
+        srv = serve.Server(factory, workers=a.batch_threads)
     out = srv.run(arrivals)
     out.update({"arrival_rate_per_s": a.serve, "window_s": a.serve_window, "max_steps": a.serve_max_steps,
-                "chi_cap": a.serve_chi_cap, "big_fraction": a.serve_big_fraction, "workers": a.batch_threads,
-                "report_every_steps": 10})
+                "chi_cap": a.serve_chi_cap, "big_fraction": a.serve_big_fraction, "report_every_steps": 10,
+                "big_requests": sum(1 for _, r in arrivals if r.inst.ncells == 101)})
     log(f"served window: p50 {out['p50_latency_s']:.2f} s p99 {out['p99_latency_s']:.2f} s, "
         f"{out['site_updates_per_s']:.0f} site-updates/s")
     return out
@@ -367,8 +378,10 @@ def main() -> int:
     ap.add_argument("--mixed-krylov-tol", type=float, default=1e-10,
                     help="Krylov exponential tolerance of the mixed engine (the plain one uses 1e-14); "
                          "its complex64 matvecs are accurate to ~1e-7")
-    ap.add_argument("--batch128", type=int, default=16,
-                    help="instances of the batched chi=128 leg (0: skip)")
+    ap.add_argument("--batch128", type=int, default=64,
+                    help="instances of the batched chi=128 leg, stepped as one group (0: skip)")
+    ap.add_argument("--batch128-streams", type=int, default=16,
+                    help="instances of the stream-per-instance comparison of that leg (0: skip)")
     ap.add_argument("--batch128-warmup", type=int, default=1)
     ap.add_argument("--batch128-steps", type=int, default=2)
     ap.add_argument("--batch", type=int, default=0,
@@ -383,6 +396,7 @@ def main() -> int:
     ap.add_argument("--serve-max-steps", type=int, default=10)
     ap.add_argument("--serve-chi-cap", type=int, default=0, help="cap on chi_max of served requests (0: none)")
     ap.add_argument("--serve-big-fraction", type=float, default=0.1)
+    ap.add_argument("--serve-method", choices=["group", "streams"], default="group")
     ap.add_argument("--tdvp1", type=int, default=0,
                     help="also time N steps of the 1TDVP chi=256 workload (101 cells), both precisions")
     ap.add_argument("--exact-apply", type=int, default=0,
@@ -390,6 +404,9 @@ def main() -> int:
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the batched workload (--batch N instances) and print its object "
                          "(the command the one-launch Krylov profile is taken with)")
+    ap.add_argument("--group128-only", action="store_true",
+                    help="prepare the chi=128 state, run only the grouped batched leg (--batch128 instances) "
+                         "and print its object (the command the grouped-step profile is taken with)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the H_eff micro-benchmark and print its roofline object "
                          "(the command the kernel profiles are taken with)")
@@ -433,6 +450,15 @@ def run(torch, qn, a) -> int:
         return 0
     if a.batch_only:
         print(json.dumps({"batched_draws": batch_bench(torch, a)}), flush=True)
+        return 0
+    if a.group128_only:
+        e = qn.Engine(mpo().W, initial_state().A, algorithm=2, precision=0, chi_max=CHI, dt=DT, eps=EPS,
+                      krylov_tol=1e-14)
+        e.step(PREP_STEPS)
+        prepared = e.tensors()
+        e.close()
+        a.batch128_streams = 0
+        print(json.dumps({"batched_chi128": batch128_bench(torch, a, prepared, float("nan"))}), flush=True)
         return 0
     Ws = mpo().W
     psi = initial_state()

@@ -16,8 +16,10 @@
 // accuracy of the one-sided method: working precision relative to the largest
 // singular value, which the 1e-10 truncation tail test needs.  A sweep without
 // rotations sets a device flag that turns the remaining launches into no-ops.
-// Columns below 1e-15 of the matrix norm are left alone (their contribution to
-// any tail is < 1e-14).  jac_finalize sorts the singular values and evaluates
+// A pair whose smaller column is below 16 eps of the matrix norm is left alone
+// (JAC_NOISE2): such columns are rounding noise of the rotations, their
+// contribution to any tail is < 4e-15, and rotating them against the signal
+// columns of a matrix with more columns than rows would go on indefinitely.  jac_finalize sorts the singular values and evaluates
 // the truncation rank (first i with ||s[i:]|| < eps, capped; reference
 // algorithms/tdvp.py:289-292).
 //
@@ -38,6 +40,11 @@ constexpr int JAC_BS = 16;           // block size: pairs of 32 columns
 constexpr int JAC_LMAX = 2 * JAC_BS;
 constexpr int JAC_RC = 64;           // rows per Gram / apply chunk
 constexpr int JAC_MAX_SWEEPS = 60;
+// Noise floor of a column, squared and relative to ||X||_F^2: (16 eps)^2.
+// Columns below 16 eps ||X|| (3.6e-15 ||X||) are not orthogonalised further;
+// their singular values are at the accuracy floor of the method, far below any
+// truncation cutoff the engine is used with (1e-10).
+constexpr double JAC_NOISE2 = 256.0 * 1.1102230246251565e-16 * 1.1102230246251565e-16;
 
 __device__ __forceinline__ double wsum(double v) {
 #pragma unroll
@@ -181,7 +188,10 @@ __device__ int inner_sweep(c128* G, c128* J, JRot (*rot)[JAC_LMAX / 2], int L, i
         const double a = G[i * L + i].re, b = G[j * L + j].re;
         const c128 gij = G[j * L + i];
         const double ag2 = gij.re * gij.re + gij.im * gij.im;
-        const int act = !(ag2 <= tol2 * a * b || ag2 < 1e-300 || fmax(a, b) < negl);
+        // a pair with a column at the noise floor is done: rotating noise
+        // against a signal column only re-creates noise, and for matrices with
+        // more columns than rows such pairs can rotate indefinitely
+        const int act = !(ag2 <= tol2 * a * b || ag2 < 1e-300 || fmin(a, b) < negl);
         JRot q{1.0, 0.0, 0.0, act, 0};
         if (act) {
           const double d = b - a;
@@ -237,7 +247,7 @@ __device__ __forceinline__ void jac_inner_body(const JacobiGeom& g, int sweep, c
   __shared__ JRot rot[2][HM];
   __shared__ int total;
   const int L = 2 * g.bs, H = g.bs, tid = threadIdx.x;
-  const double negl = 1e-30 * ctrl->fro;
+  const double negl = JAC_NOISE2 * ctrl->fro;
   for (int e = tid; e < L * L; e += JAC_NT) {
     double re = 0, im = 0;
     for (int kc = 0; kc < g.kspl; ++kc) {
@@ -421,7 +431,7 @@ __device__ __forceinline__ void jac_small_body(c128* __restrict__ X, int rows, i
   __syncthreads();
   double fro = 0;
   for (int w = 0; w < JAC_NT / 64; ++w) fro += red[w];
-  const double negl = 1e-30 * fro;
+  const double negl = JAC_NOISE2 * fro;
   const JacobiGeom g{rows, cols, H, 2, 1};  // one block pair: columns 0..H-1 and H..L-1
   int sweep = 0, converged = 0;
   for (; sweep < JAC_MAX_SWEEPS; ++sweep) {

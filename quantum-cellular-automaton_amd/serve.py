@@ -180,6 +180,136 @@ class Server:
                 "site_updates_per_s": self.site_updates / wall if wall > 0 else 0.0}
 
 
+class GroupServer:
+    """Continuous batching through grouped lockstep steps.
+
+    A scheduler loop runs one tick per time step: arrivals that came in since
+    the last tick join the pending queue (arrival order), pending requests are
+    admitted while their footprint fits the budget, and every admitted request
+    advances one time step in ONE grouped step of all of them (qca_group_step:
+    each kernel of a sweep position runs once for all instances).  A request
+    reports <n_i> every ``report_every`` steps and after its last one, then
+    leaves the group, which is rebuilt whenever its membership changes.
+    ``group_factory(engines)`` builds the group (default qca_native.Group), so
+    the scheduler is tested on the host with stand-ins."""
+
+    def __init__(self, make_engine, budget_bytes: int | None = None, report_every: int = 10,
+                 precision: int = 1, device: int = 0, group_factory=None):
+        self.make_engine = make_engine
+        if budget_bytes is None:
+            import qca_native as qn
+            free, _ = qn.device_memory(device)
+            budget_bytes = max(0, free - RESERVE_BYTES)
+        self.budget = budget_bytes
+        self.report_every = report_every
+        self.precision = precision
+        self._footprint = getattr(make_engine, "footprint", None)
+        if group_factory is None:
+            import qca_native as qn
+            group_factory = qn.Group
+        self.group_factory = group_factory
+        self.in_use = 0
+        self.peak_in_use = 0
+        self.peak_active = 0
+        self.group_builds = 0
+        self.ticks = 0
+        self.finished: list = []
+        self.site_updates = 0
+
+    def _size(self, req) -> int:
+        return (self._footprint(req.inst) if self._footprint is not None
+                else footprint_bytes(req.inst, self.precision))
+
+    def run(self, arrivals: list) -> dict:
+        arrived = collections.deque()
+        lock = threading.Lock()
+        t0 = time.perf_counter()
+        feeding = {"done": False}
+
+        def feed():
+            for offset, req in arrivals:
+                delay = t0 + offset - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                req.arrival = time.perf_counter()
+                with lock:
+                    arrived.append(req)
+            feeding["done"] = True
+
+        feeder = threading.Thread(target=feed, daemon=True)
+        feeder.start()
+        pending = collections.deque()
+        active: list = []
+        group, members = None, ()
+        try:
+            while True:
+                with lock:
+                    while arrived:
+                        req = arrived.popleft()
+                        req.footprint = self._size(req)
+                        if req.footprint > self.budget:
+                            raise ValueError(f"request {req.rid} needs {req.footprint} B, above the budget")
+                        pending.append(req)
+                while pending and self.in_use + pending[0].footprint <= self.budget:
+                    req = pending.popleft()
+                    req.engine = self.make_engine(req.inst)
+                    self.in_use += req.footprint
+                    self.peak_in_use = max(self.peak_in_use, self.in_use)
+                    active.append(req)
+                if not active:
+                    if feeding["done"] and not pending:
+                        with lock:
+                            if not arrived:
+                                break
+                    time.sleep(0.001)
+                    continue
+                key = tuple(r.rid for r in active)
+                if key != members:
+                    if group is not None:
+                        group.close()
+                    group = self.group_factory([r.engine for r in active])
+                    members = key
+                    self.group_builds += 1
+                self.peak_active = max(self.peak_active, len(active))
+                group.step(1)
+                self.ticks += 1
+                still = []
+                for r in active:
+                    r.done_steps += 1
+                    self.site_updates += r.inst.site_updates_per_step
+                    last = r.done_steps >= r.steps
+                    if last or r.done_steps % self.report_every == 0:
+                        r.reports.append(np.asarray(r.engine.rho()[:, 1, 1].real))
+                    if last:
+                        r.finished = time.perf_counter()
+                        self.finished.append(r)
+                    else:
+                        still.append(r)
+                if len(still) != len(active):
+                    group.close()
+                    group, members = None, ()
+                    for r in active:
+                        if r.finished is not None:
+                            r.engine.close()
+                            r.engine = None
+                            self.in_use -= r.footprint
+                active = still
+        finally:
+            if group is not None:
+                group.close()
+            feeder.join()
+        wall = time.perf_counter() - t0
+        lat = np.array([r.finished - r.arrival for r in self.finished])
+        return {"requests": len(self.finished), "wall_s": wall,
+                "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
+                "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
+                "site_updates": self.site_updates,
+                "site_updates_per_s": self.site_updates / wall if wall > 0 else 0.0,
+                "ticks": self.ticks, "group_builds": self.group_builds, "peak_active": self.peak_active,
+                "peak_footprint_bytes": self.peak_in_use, "budget_bytes": self.budget,
+                "method": "grouped lockstep ticks (qca_group_step), one time step per tick"}
+
+
 def poisson_arrivals(rate: float, window_s: float, seed: int = 5, big_fraction: float = 0.1,
                      max_steps: int | None = None) -> list:
     """(offset_s, Request) pairs: exponential inter-arrival times of mean

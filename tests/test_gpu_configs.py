@@ -222,6 +222,30 @@ def test_served_window_with_large_request_equals_alone(gpu):
         e.close()
 
 
+def test_group_server_window_with_large_request_equals_alone(gpu):
+    """The same served window through grouped continuous batching
+    (serve.GroupServer: one qca_group_step per tick for all admitted
+    requests): every request reports exactly what it reports alone."""
+    import serve
+    import workloads
+    arrivals = serve.poisson_arrivals(rate=50.0, window_s=0.2, seed=5, big_fraction=0.0, max_steps=12)
+    rng = np.random.default_rng(7)
+    big = workloads.Instance(101, 256, 2, range(2, 4), [1.0 if x < 0.3 else 0.0 for x in rng.random(101)])
+    arrivals[1][1].inst = big
+    make = serve.gpu_engine_factory(precision=1)
+    srv = serve.GroupServer(make)
+    out = srv.run(arrivals)
+    assert out["requests"] == len(arrivals) and out["group_builds"] >= 1
+    assert 0 < srv.peak_in_use <= srv.budget
+    for _, r in arrivals:
+        e = make(r.inst)
+        assert len(r.reports) == -(-r.steps // 10)
+        for k, rep in enumerate(r.reports):
+            e.step(min(10, r.steps - 10 * k))
+            assert np.array_equal(e.rho()[:, 1, 1].real, rep)
+        e.close()
+
+
 # ------------------------------------------------------------------ observables
 
 def test_observables_match_oracle_and_reference_golden(gpu):

@@ -293,3 +293,41 @@ def test_engine_footprint_is_host_computed_and_ordered():
     assert fac.footprint(big) < serve.gpu_engine_factory(precision=1).footprint(big)
     with pytest.raises(qn.QcaError, match="max bond"):
         qn.engine_footprint(Ws, ch, algorithm=2, precision=1, chi_max=300)
+
+
+def test_group_server_schedules_every_request():
+    """Grouped continuous batching on the host (stand-in engines and group):
+    every request advances one step per tick while admitted, reports every 10
+    steps and after its last step, admission keeps the footprint within the
+    budget, and the group is rebuilt when its membership changes."""
+    import threading
+    import serve
+    _FakeEngine.lock = threading.Lock()
+    _FakeEngine.live = _FakeEngine.peak = 0
+
+    class FakeGroup:
+        built = []
+
+        def __init__(self, engines):
+            self.engines = list(engines)
+            FakeGroup.built.append(len(self.engines))
+
+        def step(self, k):
+            for e in self.engines:
+                e.step(k)
+
+        def close(self):
+            pass
+    arrivals = serve.poisson_arrivals(rate=300.0, window_s=0.1, seed=5, max_steps=25)
+    assert len(arrivals) > 10
+    sizes = sorted(serve.footprint_bytes(r.inst) for _, r in arrivals)
+    budget = sizes[-1] + sizes[-2] + sizes[-3]
+    srv = serve.GroupServer(lambda inst: _FakeEngine(inst, 1e-5), budget_bytes=budget, group_factory=FakeGroup)
+    out = srv.run(arrivals)
+    assert out["requests"] == len(arrivals)
+    for _, r in arrivals:
+        assert len(r.reports) == -(-r.steps // 10)
+        assert r.reports[-1][0] == r.steps and r.engine is None
+    assert out["site_updates"] == sum(r.steps * r.inst.site_updates_per_step for _, r in arrivals)
+    assert _FakeEngine.live == 0 and srv.in_use == 0 and 0 < srv.peak_in_use <= budget
+    assert out["group_builds"] == len(FakeGroup.built) >= 2 and max(FakeGroup.built) >= 2
