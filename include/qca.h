@@ -211,6 +211,21 @@ int qca_group_destroy(qca_group* g);
 int qca_heff_apply(int precision, int kind, const void* x, void* y, const void* L, const void* R,
                    const double* W_host, int Din, int Dout, int chil, int chir, void* stream);
 
+/* qca_heff_apply without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and result as qca_heff_apply (the same GEMM plans, so the
+ * output is identical); its sparse MPO form and contraction workspace live in
+ * a static device array of 64 MiB per device, so an application whose
+ * workspace exceeds it (e.g. the complex128 two-site block at chi = 128,
+ * D = 13) is rejected with QCA_ERR_ARG.  The MPO form is copied in on every
+ * call, which returns after that copy; the kernels are queued on the stream.
+ * All calls on one device share the array: calls that overlap on one device
+ * (different streams or threads) race. */
+int qca_heff_apply_noalloc(int precision, int kind, const void* x, void* y, const void* L,
+                           const void* R, const double* W_host, int Din, int Dout, int chil,
+                           int chir, void* stream);
+
 /* New environment layer.  Replaces reference tdvp.py:328-347
  * (_assemble_new_layer_H_eff).  side 0 = left: Eprev (chil,Dl,chil) ->
  * Enew (chir,Dr,chir); side 1 = right: Eprev (chir,Dr,chir) -> Enew (chil,Dl,chil).

@@ -88,7 +88,7 @@ struct Scratch {
 };
 std::map<int, Scratch> g_scratch;
 
-Scratch& scratch() {
+Scratch& prim_scratch() {
   int dev = 0;
   QCA_HIP(hipGetDevice(&dev));
   return g_scratch[dev];
@@ -96,7 +96,7 @@ Scratch& scratch() {
 
 template <typename T>
 T* sbuf(const char* name, long n) {
-  return reinterpret_cast<T*>(scratch().get(name, sizeof(T) * (size_t)std::max<long>(n, 1)));
+  return reinterpret_cast<T*>(prim_scratch().get(name, sizeof(T) * (size_t)std::max<long>(n, 1)));
 }
 
 template <typename T>
@@ -529,7 +529,7 @@ int qca_heff_apply(int precision, int kind, const void* x, void* y, const void* 
     hipStream_t s = as_stream(stream);
     const int dq = kind == 0 ? 4 : (kind == 1 ? 2 : 1);
     const long tsize = (long)std::max(dq, 1) * std::max(Din, Dout) * chil * chir;
-    qca::WSparsePair* wp = kind == 2 ? nullptr : &scratch().mpo(W_host, dq, Din, Dout, false, dq * Din);
+    qca::WSparsePair* wp = kind == 2 ? nullptr : &prim_scratch().mpo(W_host, dq, Din, Dout, false, dq * Din);
     if (precision == 0) {
       auto ws = contract_ws<double>(tsize, (long)dq * chil * chir);
       qca::heff_apply<double>(kind, (const qca::c128*)x, (qca::c128*)y, (const qca::c128*)L,
@@ -554,7 +554,7 @@ int qca_env_update(int precision, int side, const void* A, const void* Eprev, vo
     QCA_CHECK(Dl >= 1 && Dr >= 1 && Dl <= 16 && Dr <= 16, "MPO bond out of range");
     hipStream_t s = as_stream(stream);
     const bool left = side == 0;
-    qca::WSparsePair& wp = scratch().mpo(W_host, 2, Dl, Dr, !left, left ? 2 * Dl : 2 * Dr);
+    qca::WSparsePair& wp = prim_scratch().mpo(W_host, 2, Dl, Dr, !left, left ? 2 * Dl : 2 * Dr);
     const int Din = left ? Dl : Dr, Dout = left ? Dr : Dl;
     const long tsize = 2L * std::max(Dl, Dr) * chil * chir;
     const long osize = (long)std::max(chil, chir) * std::max(Dl, Dr) * std::max(chil, chir);
@@ -584,7 +584,7 @@ int qca_krylov_expm(int precision, int kind, void* x, const void* L, const void*
     const int dq = kind == 0 ? 4 : (kind == 1 ? 2 : 1);
     const long n = (long)dq * chil * chir;
     const long tsize = (long)dq * std::max(Din, Dout) * chil * chir;
-    qca::WSparsePair* wp = kind == 2 ? nullptr : &scratch().mpo(W_host, dq, Din, Dout, false, dq * Din);
+    qca::WSparsePair* wp = kind == 2 ? nullptr : &prim_scratch().mpo(W_host, dq, Din, Dout, false, dq * Din);
     qca::LanczosCtrl* host = nullptr;
     qca::LanczosWork lw = lanczos_ws(n, &host);
     const double tau = 1.5707963267948966 * delta;
@@ -781,10 +781,116 @@ int qca_exact_rho(const void* psi, int ncells, double* rho_host, void* stream) {
   });
 }
 
+}  // extern "C"
+
+// ------------------------------------------------------------------ no-allocation variant
+// Static device scratch of qca_heff_apply_noalloc: the sparse MPO form and the
+// contraction workspace (both intermediates and the split-K partials) of one
+// application, in the layout and sizes qca_heff_apply allocates, so both
+// entry points run the same GEMM plans.  One copy per device, held by the
+// loaded code object until the process ends.
+constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+std::mutex g_noalloc_mu;
+std::map<int, unsigned char*> g_noalloc_ptr;   // device -> address of `scratch` there
+
+// Device address of `scratch` on the current device (looked up once per
+// device with that device current; the array's name is only a host stand-in).
+unsigned char* noalloc_scratch() {
+  int dev = 0;
+  QCA_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  auto it = g_noalloc_ptr.find(dev);
+  if (it != g_noalloc_ptr.end()) return it->second;
+  void* p = nullptr;
+  QCA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  QCA_CHECK(p != nullptr, "no device address for the static scratch");
+  g_noalloc_ptr[dev] = reinterpret_cast<unsigned char*>(p);
+  return reinterpret_cast<unsigned char*>(p);
+}
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+template <typename T>
+void heff_noalloc(int kind, const void* x, void* y, const void* L, const void* R, const double* W_host,
+                  int Din, int Dout, int chil, int chir, hipStream_t s) {
+  const int dq = kind == 0 ? 4 : (kind == 1 ? 2 : 1);
+  const long tsize = (long)std::max(dq, 1) * std::max(Din, Dout) * chil * chir;
+  const long outsize = (long)dq * chil * chir;
+  const long splitk_elems = 16 * std::max(outsize, tsize / 4);   // as contract_ws
+  qca::HostSparse hs;
+  size_t mpo_bytes = 0;
+  if (kind != 2) {
+    hs = qca::build_sparse(reinterpret_cast<const qca::c128*>(W_host), dq, Din, Dout, false);
+    mpo_bytes = align256(sizeof(int) * hs.rowptr.size()) + align256(sizeof(int) * std::max<size_t>(1, hs.col.size())) +
+                align256(sizeof(qca::cplx<T>) * std::max<size_t>(1, hs.val.size()));
+  }
+  const size_t need = mpo_bytes + 2 * align256(sizeof(qca::cplx<T>) * tsize) +
+                      align256(sizeof(qca::cplx<T>) * splitk_elems);
+  if (need > kNoAllocScratchBytes)
+    throw qca::Error(QCA_ERR_ARG, "qca_heff_apply_noalloc: this application needs " + std::to_string(need) +
+                                      " bytes of workspace, above the 64 MiB static scratch");
+  unsigned char* base = noalloc_scratch();
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    unsigned char* p = base + off;
+    off += align256(bytes);
+    return p;
+  };
+  qca::WSparseDev<T> w;
+  if (kind != 2) {
+    w.rowptr = reinterpret_cast<int*>(take(sizeof(int) * hs.rowptr.size()));
+    w.col = reinterpret_cast<int*>(take(sizeof(int) * std::max<size_t>(1, hs.col.size())));
+    w.val = reinterpret_cast<qca::cplx<T>*>(take(sizeof(qca::cplx<T>) * std::max<size_t>(1, hs.val.size())));
+    w.nout = (int)hs.rowptr.size() - 1;
+    w.nin = dq * Din;
+    w.nnz = (int)hs.col.size();
+    std::vector<qca::cplx<T>> val(hs.val.size());
+    for (size_t z = 0; z < val.size(); ++z) val[z] = qca::cplx<T>{(T)hs.val[z].re, (T)hs.val[z].im};
+    // pageable sources: the copies are complete when the calls return
+    QCA_HIP(hipMemcpyAsync(w.rowptr, hs.rowptr.data(), sizeof(int) * hs.rowptr.size(), hipMemcpyHostToDevice, s));
+    if (!hs.col.empty()) {
+      QCA_HIP(hipMemcpyAsync(w.col, hs.col.data(), sizeof(int) * hs.col.size(), hipMemcpyHostToDevice, s));
+      QCA_HIP(hipMemcpyAsync(w.val, val.data(), sizeof(qca::cplx<T>) * val.size(), hipMemcpyHostToDevice, s));
+    }
+    QCA_HIP(hipStreamSynchronize(s));
+  }
+  qca::ContractWs<T> ws;
+  ws.T1 = reinterpret_cast<qca::cplx<T>*>(take(sizeof(qca::cplx<T>) * tsize));
+  ws.U = reinterpret_cast<qca::cplx<T>*>(take(sizeof(qca::cplx<T>) * tsize));
+  ws.splitk_elems = splitk_elems;
+  ws.splitk = reinterpret_cast<qca::cplx<T>*>(take(sizeof(qca::cplx<T>) * splitk_elems));
+  qca::heff_apply<T>(kind, reinterpret_cast<const qca::cplx<T>*>(x), reinterpret_cast<qca::cplx<T>*>(y),
+                     reinterpret_cast<const qca::cplx<T>*>(L), reinterpret_cast<const qca::cplx<T>*>(R),
+                     kind == 2 ? nullptr : &w, Din, Dout, chil, chir, ws, nullptr, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int qca_heff_apply_noalloc(int precision, int kind, const void* x, void* y, const void* L, const void* R,
+                           const double* W_host, int Din, int Dout, int chil, int chir, void* stream) {
+  return guarded([&] {
+    QCA_CHECK(precision == 0 || precision == 1, "precision must be 0 or 1");
+    QCA_CHECK(kind >= 0 && kind <= 2, "kind must be 0, 1 or 2");
+    QCA_CHECK(x && y && L && R, "null tensor");
+    QCA_CHECK(chil >= 1 && chir >= 1 && chil <= 512 && chir <= 512, "bond out of range");
+    QCA_CHECK(Din >= 1 && Dout >= 1 && Din <= 16 && Dout <= 16, "MPO bond out of range");
+    QCA_CHECK(kind == 2 ? Din == Dout : W_host != nullptr, "bad MPO arguments");
+    hipStream_t s = as_stream(stream);
+    if (precision == 0) heff_noalloc<double>(kind, x, y, L, R, W_host, Din, Dout, chil, chir, s);
+    else heff_noalloc<float>(kind, x, y, L, R, W_host, Din, Dout, chil, chir, s);
+  });
+}
+
 int qca_release_workspace(void) {
   return guarded([&] {
     QCA_HIP(hipDeviceSynchronize());
-    scratch().release();
+    prim_scratch().release();
   });
 }
 
