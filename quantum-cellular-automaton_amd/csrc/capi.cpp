@@ -825,10 +825,11 @@ void heff_noalloc(int kind, const void* x, void* y, const void* L, const void* R
   size_t mpo_bytes = 0;
   if (kind != 2) {
     hs = qca::build_sparse(reinterpret_cast<const qca::c128*>(W_host), dq, Din, Dout, false);
-    mpo_bytes = align256(sizeof(int) * hs.rowptr.size()) + align256(sizeof(int) * std::max<size_t>(1, hs.col.size())) +
+    mpo_bytes = align256(sizeof(int) * hs.rowptr.size()) + 2 * align256(sizeof(int) * std::max<size_t>(1, hs.col.size())) +
                 align256(sizeof(qca::cplx<T>) * std::max<size_t>(1, hs.val.size()));
   }
-  const size_t need = mpo_bytes + 2 * align256(sizeof(qca::cplx<T>) * tsize) +
+  // the effective-Hamiltonian chain needs T1 and the split-K partials only
+  const size_t need = mpo_bytes + align256(sizeof(qca::cplx<T>) * tsize) +
                       align256(sizeof(qca::cplx<T>) * splitk_elems);
   if (need > kNoAllocScratchBytes)
     throw qca::Error(QCA_ERR_ARG, "qca_heff_apply_noalloc: this application needs " + std::to_string(need) +
@@ -844,23 +845,26 @@ void heff_noalloc(int kind, const void* x, void* y, const void* L, const void* R
   if (kind != 2) {
     w.rowptr = reinterpret_cast<int*>(take(sizeof(int) * hs.rowptr.size()));
     w.col = reinterpret_cast<int*>(take(sizeof(int) * std::max<size_t>(1, hs.col.size())));
+    w.outv = reinterpret_cast<int*>(take(sizeof(int) * std::max<size_t>(1, hs.outv.size())));
     w.val = reinterpret_cast<qca::cplx<T>*>(take(sizeof(qca::cplx<T>) * std::max<size_t>(1, hs.val.size())));
     w.nout = (int)hs.rowptr.size() - 1;
     w.nin = dq * Din;
     w.nnz = (int)hs.col.size();
+    w.group_rows = hs.group_rows;
+    w.group_max = hs.group_max;
     std::vector<qca::cplx<T>> val(hs.val.size());
     for (size_t z = 0; z < val.size(); ++z) val[z] = qca::cplx<T>{(T)hs.val[z].re, (T)hs.val[z].im};
     // pageable sources: the copies are complete when the calls return
     QCA_HIP(hipMemcpyAsync(w.rowptr, hs.rowptr.data(), sizeof(int) * hs.rowptr.size(), hipMemcpyHostToDevice, s));
     if (!hs.col.empty()) {
       QCA_HIP(hipMemcpyAsync(w.col, hs.col.data(), sizeof(int) * hs.col.size(), hipMemcpyHostToDevice, s));
+      QCA_HIP(hipMemcpyAsync(w.outv, hs.outv.data(), sizeof(int) * hs.outv.size(), hipMemcpyHostToDevice, s));
       QCA_HIP(hipMemcpyAsync(w.val, val.data(), sizeof(qca::cplx<T>) * val.size(), hipMemcpyHostToDevice, s));
     }
     QCA_HIP(hipStreamSynchronize(s));
   }
   qca::ContractWs<T> ws;
   ws.T1 = reinterpret_cast<qca::cplx<T>*>(take(sizeof(qca::cplx<T>) * tsize));
-  ws.U = reinterpret_cast<qca::cplx<T>*>(take(sizeof(qca::cplx<T>) * tsize));
   ws.splitk_elems = splitk_elems;
   ws.splitk = reinterpret_cast<qca::cplx<T>*>(take(sizeof(qca::cplx<T>) * splitk_elems));
   qca::heff_apply<T>(kind, reinterpret_cast<const qca::cplx<T>*>(x), reinterpret_cast<qca::cplx<T>*>(y),

@@ -1,6 +1,7 @@
 // Contraction launch sequences of the TDVP path; see contract.hpp.
 #include "contract.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <type_traits>
 
@@ -56,6 +57,8 @@ ContractPlan<T> heff_plan(int kind, const cplx<T>* x, cplx<T>* y, const cplx<T>*
     return p;
   }
   const int dq = kind == 0 ? 4 : 2;
+  QCA_CHECK(W->group_rows == Dout && W->nout == dq * Dout && W->nin == dq * Din,
+            "sparse MPO does not match the block");
   // GEMM1: T[q][(w,x)][c] = sum_a L[a][(w,x)] x[q][a][c]
   GemmDesc g = gemm_desc(Din * chil, chir, chil);
   g.NB2 = dq;
@@ -64,17 +67,25 @@ ContractPlan<T> heff_plan(int kind, const cplx<T>* x, cplx<T>* y, const cplx<T>*
   g.c_m = chir; g.c_n = 1; g.c_b2 = (long)Din * lr;
   g.skip = skip;
   p.g1 = GemmJob<T>{g, L, x, ws.T1, ws.splitk, ws.splitk_elems};
-  // U[q'][(x,c)][v] = sum_{q,w} W[q][q'][w][v] T[q][w][(x,c)]
-  p.mix = true;
-  p.wm = WmixArgs<T>{ws.T1, lr, lr, dq * Din, ws.U, dq, Dout, W->rowptr, W->col, W->val, skip};
-  // GEMM3: y[q'][x][y] = sum_{(c,v)} U[q'][x][(c,v)] R[(c,v)][y]
-  GemmDesc h = gemm_desc(chil, chir, chir * Dout);
+  // GEMM3 with the MPO mixing gathered into its K loop:
+  //   y[q'][x][y] = sum_{(q,w,v) in W[.][q'][.][.]} W[q][q'][w][v] sum_c T[q][w][x][c] R[c][v][y]
+  // one K segment per nonzero of the column block q' (seg_col = q*Din + w,
+  // seg_outv = v), each a sum over c.
+  p.mix = false;
+  GemmDesc h = gemm_desc(chil, chir, W->group_max * chir);
   h.NB2 = dq;
-  h.a_m = (long)chir * Dout; h.a_k2 = 1; h.a_b2 = lr * Dout;
-  h.b_k2 = chir; h.b_n = 1;
+  h.K2 = chir;
+  h.seg_rows = Dout;
+  h.seg_max = W->group_max;
+  h.seg_rowptr = W->rowptr;
+  h.seg_col = W->col;
+  h.seg_outv = W->outv;
+  h.seg_val = W->val;
+  h.a_m = chir; h.a_k1 = lr; h.a_k2 = 1;
+  h.b_k1 = chir; h.b_k2 = (long)Dout * chir; h.b_n = 1;
   h.c_m = chir; h.c_n = 1; h.c_b2 = lr;
   h.skip = skip;
-  p.g3 = GemmJob<T>{h, ws.U, R, y, ws.splitk, ws.splitk_elems};
+  p.g3 = GemmJob<T>{h, ws.T1, R, y, ws.splitk, ws.splitk_elems};
   return p;
 }
 
@@ -190,13 +201,17 @@ HostSparse build_sparse(const c128* W, int dq, int Dl, int Dr, bool backward) {
           if (!backward) rows[q * Dr + v].push_back({p * Dl + w, x});
           else rows[p * Dl + w].push_back({q * Dr + v, x});
         }
+  h.group_rows = backward ? Dl : Dr;
   for (int o = 0; o < nout; ++o) {
     for (auto& e : rows[o]) {
       h.col.push_back(e.first);
       h.val.push_back(e.second);
+      h.outv.push_back(o % h.group_rows);
     }
     h.rowptr[o + 1] = (int)h.col.size();
   }
+  for (int g = 0; g < dq; ++g)
+    h.group_max = std::max(h.group_max, h.rowptr[(g + 1) * h.group_rows] - h.rowptr[g * h.group_rows]);
   return h;
 }
 
@@ -229,7 +244,7 @@ size_t sparse_alloc_size(size_t bytes) { return bytes < 4 ? 4 : bytes; }
 
 size_t sparse_device_bytes(const HostSparse& h) {
   const size_t nout = h.rowptr.size() - 1, nnz = h.col.size();
-  return sparse_alloc_size(sizeof(int) * (nout + 1)) + sparse_alloc_size(sizeof(int) * nnz) +
+  return sparse_alloc_size(sizeof(int) * (nout + 1)) + 2 * sparse_alloc_size(sizeof(int) * nnz) +
          sparse_alloc_size(sizeof(c128) * nnz) + sparse_alloc_size(sizeof(c64) * nnz);
 }
 
@@ -244,11 +259,13 @@ void WSparsePair::upload(const HostSparse& h, int nin) {
   };
   d64.rowptr = (int*)alloc(sizeof(int) * (nout + 1));
   d64.col = (int*)alloc(sizeof(int) * nnz);
+  d64.outv = (int*)alloc(sizeof(int) * nnz);
   d64.val = (c128*)alloc(sizeof(c128) * nnz);
   d32.val = (c64*)alloc(sizeof(c64) * nnz);
   QCA_HIP(hipMemcpy(d64.rowptr, h.rowptr.data(), sizeof(int) * (nout + 1), hipMemcpyHostToDevice));
   if (nnz) {
     QCA_HIP(hipMemcpy(d64.col, h.col.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+    QCA_HIP(hipMemcpy(d64.outv, h.outv.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
     QCA_HIP(hipMemcpy(d64.val, h.val.data(), sizeof(c128) * nnz, hipMemcpyHostToDevice));
     std::vector<c64> v32(nnz);
     for (int z = 0; z < nnz; ++z) v32[z] = c64{(float)h.val[z].re, (float)h.val[z].im};
@@ -256,6 +273,9 @@ void WSparsePair::upload(const HostSparse& h, int nin) {
   }
   d32.rowptr = d64.rowptr;
   d32.col = d64.col;
+  d32.outv = d64.outv;
+  d64.group_rows = d32.group_rows = h.group_rows;
+  d64.group_max = d32.group_max = h.group_max;
   d64.nout = d32.nout = nout;
   d64.nin = d32.nin = nin;
   d64.nnz = d32.nnz = nnz;

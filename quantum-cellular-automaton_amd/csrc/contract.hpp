@@ -67,6 +67,9 @@ double heff_flops(int kind, long chil, long chir, long D);
 struct HostSparse {
   std::vector<int> rowptr, col;
   std::vector<c128> val;
+  std::vector<int> outv;  // row % group_rows of each nonzero
+  int group_rows = 0;     // rows per output physical index
+  int group_max = 0;      // most nonzeros in any group
 };
 HostSparse build_sparse(const c128* W, int dq, int Dl, int Dr, bool backward);
 std::vector<c128> merge_mpo(const c128* Wl, const c128* Wr, int Dl, int Dm, int Dr);

@@ -9,14 +9,17 @@
 // strides (see GemmDesc) so that every TDVP contraction is a single launch with
 // no permutation copies; the load loop walks the unit-stride index so global
 // reads stay coalesced; the next K-tile is loaded into registers while the
-// current one is multiplied.  Launches with too few output tiles to fill 256
-// CUs split K and reduce the partial tiles in a second, deterministic pass.
+// current one is multiplied.  The K range is walked as segments, which also
+// lets the MPO mixing of the effective Hamiltonian be gathered into the
+// operand loads (GemmDesc's gather form) instead of a separate pass.
+// Launches with too few output tiles to fill 256 CUs split K and reduce the
+// partial tiles in a second, deterministic pass.
 #include "grouped.hpp"
 
 namespace qca {
 namespace {
 
-constexpr int BM = 64, BN = 64, BK = 16, PAD = 1;
+constexpr int BM = 64, BN = 64, BK = 16;
 constexpr int LOADS = BM * BK / 256;  // operand elements per thread per K-tile
 
 template <typename T>
@@ -43,32 +46,56 @@ struct Mfma<float> {
   static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-// KG K-groups of 256 threads share a workgroup: group g takes the K-tiles
-// g, g + KG, ... of the workgroup's range into its own LDS buffers, and the
-// groups' partial tiles are summed through LDS at the end.  With KG = 2 a CU
-// holding one workgroup runs two waves per SIMD, so one group's staging and
-// barrier waits overlap the other's MFMAs.
+// One LDS plane (real or imaginary part of one operand tile).  An operand
+// whose tile rows (m or n) are contiguous in memory is staged as [k][mn], one
+// whose k is contiguous as [mn][k].  The row strides keep both the stores of
+// the coalesced loads and the MFMA operand reads (16 consecutive mn at each of
+// 4 k per wave) free of bank conflicts: LDS serves a wave64 b32 access 32
+// lanes per cycle and a b64 access 16 lanes per cycle over 32 banks.
+template <typename T, bool MNCONT>
+struct Plane {
+  static constexpr int LS = MNCONT ? (sizeof(T) == 4 ? BM + 16 : BM) : (sizeof(T) == 4 ? BK + 2 : BK + 1);
+  static constexpr int SIZE = MNCONT ? BK * LS : BM * LS;
+  static __device__ __forceinline__ int at(int k, int mn) { return MNCONT ? k * LS + mn : mn * LS + k; }
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return cplx<T>{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+
 // One output tile (bx: N tile, by: M tile, bz: batch * splits + split) of the
 // GEMM d; the kernels below supply the block index (direct launch) or a
 // record's virtual one (grouped launch over many instances).
+//
+// The K range of the batch is a list of segments (see GemmDesc) cut into
+// K-tiles of BK (a segment's last tile is masked); split `split` takes tiles
+// [split * tiles_per_split, +tiles_per_split).  KG K-groups of 256 threads
+// share the workgroup: group g takes every KG-th tile into its own LDS
+// stages, and the groups' partial tiles are summed through LDS at the end.
+// With KG = 2 a CU runs two waves per SIMD, so one group's staging overlaps
+// the other's MFMAs.  Each group double-buffers its LDS: the tile loaded into
+// registers during iteration i-1 is stored into stage i%2 while stage
+// (i-1)%2 may still be read, so one barrier per K-tile suffices, and the
+// global loads of tile i+1 are in flight during the MFMAs of tile i.
 template <typename T, bool A_MCONT, bool B_NCONT, int KG>
 __device__ __forceinline__ void gemm_body(const GemmDesc& d, const cplx<T>* __restrict__ A,
                                           const cplx<T>* __restrict__ B, cplx<T>* __restrict__ C,
-                                          cplx<T>* __restrict__ ws, int splits, int k_per_split,
+                                          cplx<T>* __restrict__ ws, int splits, int tiles_per_split,
                                           unsigned bx, unsigned by, unsigned bz) {
   using acc_t = typename Mfma<T>::acc_t;
-  static_assert(BM == BN, "one LDS stride for both operands");
+  using PA = Plane<T, A_MCONT>;
+  using PB = Plane<T, B_NCONT>;
+  constexpr int STAGE = 2 * PA::SIZE + 2 * PB::SIZE;  // [A re, A im, B re, B im]
+  __shared__ T smem[KG * 2 * STAGE];
+  static_assert(KG == 1 || KG * 2 * STAGE >= 4 * 32 * 64, "partial-tile exchange fits the operand LDS");
   if (d.skip && *d.skip) return;
-  constexpr int LS = BM + PAD;
-  // [4 planes: A re, A im, B re, B im][KG][BK][LS]
-  __shared__ T smem[4 * KG * BK * LS];
-  static_assert(KG == 1 || 4 * KG * BK * LS >= 4 * 32 * 64, "partial-tile exchange fits the operand LDS");
-  auto lds = [&](int plane, int g, int k, int x) -> T& {
-    return smem[((plane * KG + g) * BK + k) * LS + x];
-  };
 
   const int tid = threadIdx.x & 255;  // thread within its K-group
-  const int grp = threadIdx.x >> 8;
+  // the K-group is wave-uniform: keeping it (and so the tile schedule, the
+  // segment lookups and the tile base addresses) in scalar registers
+  const int grp = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8);
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = (wave >> 1) * 32;
@@ -80,11 +107,41 @@ __device__ __forceinline__ void gemm_body(const GemmDesc& d, const cplx<T>* __re
   const int b1 = zb / d.NB2;
   const int b2 = zb - b1 * d.NB2;
 
-  const cplx<T>* Ab = A + b1 * d.a_b1 + b2 * d.a_b2;
-  const cplx<T>* Bb = B + b1 * d.b_b1 + b2 * d.b_b2;
+  const cplx<T>* Ab = A + b1 * d.a_b1 + b2 * d.a_b2 + (long)m0 * d.a_m;
+  const cplx<T>* Bb = B + b1 * d.b_b1 + b2 * d.b_b2 + (long)n0 * d.b_n;
 
-  const int k_begin = split * k_per_split;
-  const int k_end = min(d.K, k_begin + k_per_split);
+  const bool gather = d.seg_rows > 0;
+  int e0 = 0, nseg;
+  if (gather) {
+    e0 = d.seg_rowptr[b2 * d.seg_rows];
+    nseg = d.seg_rowptr[(b2 + 1) * d.seg_rows] - e0;
+  } else {
+    nseg = d.K / d.K2;
+  }
+  const int tps = (d.K2 + BK - 1) / BK;
+  const int t_begin = split * tiles_per_split;
+  const int t_end = min(nseg * tps, t_begin + tiles_per_split);
+  const int niter = t_end > t_begin ? (t_end - t_begin + KG - 1) / KG : 0;
+
+  // element offsets of this thread's loads within a tile (strides are
+  // non-negative and a tile spans less than 2^31 elements: checked on the host)
+  unsigned offa[LOADS], offb[LOADS];
+  int kla[LOADS], klb[LOADS];
+  bool oka[LOADS], okb[LOADS];
+#pragma unroll
+  for (int r = 0; r < LOADS; ++r) {
+    const int e = tid + 256 * r;
+    const int ml = A_MCONT ? (e % BM) : (e / BK);
+    const int ka = A_MCONT ? (e / BM) : (e % BK);
+    const int nl = B_NCONT ? (e % BN) : (e / BK);
+    const int kb = B_NCONT ? (e / BN) : (e % BK);
+    oka[r] = m0 + ml < d.M;
+    okb[r] = n0 + nl < d.N;
+    kla[r] = ka;
+    klb[r] = kb;
+    offa[r] = (unsigned)(ml * d.a_m + ka * d.a_k2);
+    offb[r] = (unsigned)(nl * d.b_n + kb * d.b_k2);
+  }
 
   acc_t acc_re[2][2], acc_im[2][2];
 #pragma unroll
@@ -98,108 +155,118 @@ __device__ __forceinline__ void gemm_body(const GemmDesc& d, const cplx<T>* __re
   const T sa = d.conj_a ? T(-1) : T(1);
   const T sb = d.conj_b ? T(-1) : T(1);
 
-  // Operand elements of one K-tile, loaded into registers one tile ahead so
-  // that the global loads of tile k0+BK overlap the MFMAs of tile k0.
-  const bool flatK = d.K2 >= d.K;  // single-level K: no k1/k2 split
+  // operand elements of one K-tile (and the tile's gather weight), loaded into
+  // registers one iteration ahead of their LDS store
   cplx<T> pa[LOADS], pb[LOADS];
-  auto load_tile = [&](int k0) {
+  cplx<T> wgt{T(1), T(0)};
+  auto load_tile = [&](int t) {
+    if (t >= t_end) {  // uniform per K-group: no memory access
+#pragma unroll
+      for (int r = 0; r < LOADS; ++r) pa[r] = pb[r] = cplx<T>{T(0), T(0)};
+      return;
+    }
+    const int seg = t / tps;
+    const int k0 = (t - seg * tps) * BK;
+    long soa, sob;
+    if (gather) {
+      const int e = e0 + seg;
+      soa = (long)d.seg_col[e] * d.a_k1;
+      sob = (long)d.seg_outv[e] * d.b_k1;
+      wgt = reinterpret_cast<const cplx<T>*>(d.seg_val)[e];
+    } else {
+      soa = (long)seg * d.a_k1;
+      sob = (long)seg * d.b_k1;
+    }
+    const cplx<T>* ta = Ab + soa + (long)k0 * d.a_k2;
+    const cplx<T>* tb = Bb + sob + (long)k0 * d.b_k2;
+    const int klim = d.K2 - k0;
+    // an element outside the operand reads the tile's first element (always
+    // valid) and is replaced by zero
 #pragma unroll
     for (int r = 0; r < LOADS; ++r) {
-      const int e = tid + 256 * r;
-      const int ml = A_MCONT ? (e % BM) : (e / BK);
-      const int kl = A_MCONT ? (e / BM) : (e % BK);
-      const int m = m0 + ml, k = k0 + kl;
-      cplx<T> v{T(0), T(0)};
-      if (m < d.M && k < k_end) {
-        if (flatK) {
-          v = Ab[m * d.a_m + k * d.a_k2];
-        } else {
-          const int k1 = k / d.K2, k2 = k - k1 * d.K2;
-          v = Ab[m * d.a_m + k1 * d.a_k1 + k2 * d.a_k2];
-        }
-      }
-      pa[r] = v;
+      const bool ok = oka[r] && kla[r] < klim;
+      const cplx<T> v = ta[ok ? offa[r] : 0u];
+      pa[r] = ok ? v : cplx<T>{T(0), T(0)};
     }
 #pragma unroll
     for (int r = 0; r < LOADS; ++r) {
-      const int e = tid + 256 * r;
-      const int nl = B_NCONT ? (e % BN) : (e / BK);
-      const int kl = B_NCONT ? (e / BN) : (e % BK);
-      const int n = n0 + nl, k = k0 + kl;
-      cplx<T> v{T(0), T(0)};
-      if (n < d.N && k < k_end) {
-        if (flatK) {
-          v = Bb[k * d.b_k2 + n * d.b_n];
-        } else {
-          const int k1 = k / d.K2, k2 = k - k1 * d.K2;
-          v = Bb[k1 * d.b_k1 + k2 * d.b_k2 + n * d.b_n];
-        }
-      }
-      pb[r] = v;
+      const bool ok = okb[r] && klb[r] < klim;
+      const cplx<T> v = tb[ok ? offb[r] : 0u];
+      pb[r] = ok ? v : cplx<T>{T(0), T(0)};
     }
   };
 
-  // tiles beyond k_end load as zeros (no memory access), so every group can
-  // run the same number of iterations
-  load_tile(k_begin + grp * BK);
-  for (int k0 = k_begin; k0 < k_end; k0 += KG * BK) {
-    const int kt = k0 + grp * BK;
-    // ---- registers -> LDS (re/im planes), then prefetch this group's next tile
+  auto stage_ptr = [&](int stage) { return smem + (grp * 2 + stage) * STAGE; };
+
+  load_tile(t_begin + grp);
+  for (int it = 0; it < niter; ++it) {
+    const int t = t_begin + it * KG + grp;
+    T* st = stage_ptr(it & 1);
+    T* a_re = st;
+    T* a_im = st + PA::SIZE;
+    T* b_re = st + 2 * PA::SIZE;
+    T* b_im = st + 2 * PA::SIZE + PB::SIZE;
+    // ---- registers -> LDS (re/im planes), then start this group's next tile
+    const cplx<T> w = wgt;
 #pragma unroll
     for (int r = 0; r < LOADS; ++r) {
       const int e = tid + 256 * r;
       const int ml = A_MCONT ? (e % BM) : (e / BK);
       const int kl = A_MCONT ? (e / BM) : (e % BK);
-      lds(0, grp, kl, ml) = pa[r].re;
-      lds(1, grp, kl, ml) = sa * pa[r].im;
+      const cplx<T> v = gather ? cmul(pa[r], w) : pa[r];
+      a_re[PA::at(kl, ml)] = v.re;
+      a_im[PA::at(kl, ml)] = sa * v.im;
     }
 #pragma unroll
     for (int r = 0; r < LOADS; ++r) {
       const int e = tid + 256 * r;
       const int nl = B_NCONT ? (e % BN) : (e / BK);
       const int kl = B_NCONT ? (e / BN) : (e % BK);
-      lds(2, grp, kl, nl) = pb[r].re;
-      lds(3, grp, kl, nl) = sb * pb[r].im;
+      b_re[PB::at(kl, nl)] = pb[r].re;
+      b_im[PB::at(kl, nl)] = sb * pb[r].im;
     }
+    load_tile(t + KG);
     __syncthreads();
-    if (kt + KG * BK < k_end) load_tile(kt + KG * BK);
-    else if (k0 + KG * BK < k_end) load_tile(k_end);  // this group idles in the last round: zeros
 
-    // a group without a tile in the last round skips its MFMAs (its registers
-    // hold zeros; the barriers stay outside)
-    if (kt < k_end)
-#pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-      const int kr = kk + (lane >> 4);
+    // a group without a tile in the last round skips its MFMAs (the barrier
+    // stays outside)
+    if (t < t_end) {
       const int c = lane & 15;
-      T ar[2], ai[2], br[2], bi[2];
+      const int q = lane >> 4;
+      T ar[4][2], ai[4][2], br[4][2], bi[4][2];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        ar[i] = lds(0, grp, kr, wm + 16 * i + c);
-        ai[i] = lds(1, grp, kr, wm + 16 * i + c);
-        br[i] = lds(2, grp, kr, wn + 16 * i + c);
-        bi[i] = lds(3, grp, kr, wn + 16 * i + c);
+      for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          ar[kk][i] = a_re[PA::at(4 * kk + q, wm + 16 * i + c)];
+          ai[kk][i] = a_im[PA::at(4 * kk + q, wm + 16 * i + c)];
+          br[kk][i] = b_re[PB::at(4 * kk + q, wn + 16 * i + c)];
+          bi[kk][i] = b_im[PB::at(4 * kk + q, wn + 16 * i + c)];
+        }
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            acc_re[i][j] = Mfma<T>::mac(ar[kk][i], br[kk][j], acc_re[i][j]);
+            acc_im[i][j] = Mfma<T>::mac(ar[kk][i], bi[kk][j], acc_im[i][j]);
+          }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            acc_re[i][j] = Mfma<T>::mac(-ai[kk][i], bi[kk][j], acc_re[i][j]);
+            acc_im[i][j] = Mfma<T>::mac(ai[kk][i], br[kk][j], acc_im[i][j]);
+          }
       }
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          acc_re[i][j] = Mfma<T>::mac(ar[i], br[j], acc_re[i][j]);
-          acc_im[i][j] = Mfma<T>::mac(ar[i], bi[j], acc_im[i][j]);
-        }
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          acc_re[i][j] = Mfma<T>::mac(-ai[i], bi[j], acc_re[i][j]);
-          acc_im[i][j] = Mfma<T>::mac(ai[i], br[j], acc_im[i][j]);
-        }
     }
-    __syncthreads();
   }
 
+  // group 0 writes the tile; with KG = 2 group 1 first hands it its partial
+  // tile through the operand LDS, once every wave is done reading it
   if constexpr (KG == 2) {
-    // group 1 hands its partial tile to group 0 through the (now free) operand LDS
+    __syncthreads();
     T* xs = smem + (wave * 32) * 64 + lane;  // [wave][value][lane]
     if (grp == 1) {
 #pragma unroll
@@ -213,42 +280,45 @@ __device__ __forceinline__ void gemm_body(const GemmDesc& d, const cplx<T>* __re
           }
     }
     __syncthreads();
-    if (grp == 1) return;
+    if (grp == 0) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            acc_re[i][j][r] += xs[(((i * 2 + j) * 4 + r) * 2 + 0) * 64];
+            acc_im[i][j][r] += xs[(((i * 2 + j) * 4 + r) * 2 + 1) * 64];
+          }
+    }
+  }
+  const T al_re = T(d.alpha_re), al_im = T(d.alpha_im);
+  auto store_out = [&](int m, int n, T xr, T xi) {
+    cplx<T>* cp = C + b1 * d.c_b1 + b2 * d.c_b2 + m * d.c_m + n * d.c_n;
+    cplx<T> out{al_re * xr - al_im * xi, al_re * xi + al_im * xr};
+    if (d.accumulate) {
+      out.re += cp->re;
+      out.im += cp->im;
+    }
+    *cp = out;
+  };
+  // ---- epilogue: column lane%16, rows per Mfma<T>::row of each 16x16 tile
+  if (grp == 0) {
+    const int col = lane & 15;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int j = 0; j < 2; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          acc_re[i][j][r] += xs[(((i * 2 + j) * 4 + r) * 2 + 0) * 64];
-          acc_im[i][j][r] += xs[(((i * 2 + j) * 4 + r) * 2 + 1) * 64];
+          const int m = m0 + wm + 16 * i + Mfma<T>::row(lane, r);
+          const int n = n0 + wn + 16 * j + col;
+          if (m >= d.M || n >= d.N) continue;
+          const T xr = acc_re[i][j][r], xi = acc_im[i][j][r];
+          if (splits > 1) ws[((long)bz * d.M + m) * d.N + n] = cplx<T>{xr, xi};
+          else store_out(m, n, xr, xi);
         }
   }
-  // ---- epilogue: column lane%16, rows per Mfma<T>::row of each 16x16 tile
-  const int col = lane & 15;
-  const T al_re = T(d.alpha_re), al_im = T(d.alpha_im);
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wm + 16 * i + Mfma<T>::row(lane, r);
-        const int n = n0 + wn + 16 * j + col;
-        if (m >= d.M || n >= d.N) continue;
-        const T xr = acc_re[i][j][r], xi = acc_im[i][j][r];
-        if (splits > 1) {
-          ws[((long)bz * d.M + m) * d.N + n] = cplx<T>{xr, xi};
-        } else {
-          cplx<T>* cp = C + b1 * d.c_b1 + b2 * d.c_b2 + m * d.c_m + n * d.c_n;
-          cplx<T> out{al_re * xr - al_im * xi, al_re * xi + al_im * xr};
-          if (d.accumulate) {
-            out.re += cp->re;
-            out.im += cp->im;
-          }
-          *cp = out;
-        }
-      }
 }
 
 template <typename T, bool A_MCONT, bool B_NCONT, int KG>
@@ -256,8 +326,8 @@ __global__ __launch_bounds__(256 * KG) void gemm_kernel(GemmDesc d, const cplx<T
                                                         const cplx<T>* __restrict__ B,
                                                         cplx<T>* __restrict__ C,
                                                         cplx<T>* __restrict__ ws, int splits,
-                                                        int k_per_split) {
-  gemm_body<T, A_MCONT, B_NCONT, KG>(d, A, B, C, ws, splits, k_per_split, blockIdx.x, blockIdx.y,
+                                                        int tiles_per_split) {
+  gemm_body<T, A_MCONT, B_NCONT, KG>(d, A, B, C, ws, splits, tiles_per_split, blockIdx.x, blockIdx.y,
                                      blockIdx.z);
 }
 
@@ -317,6 +387,16 @@ GemmPlan gemm_plan(const GemmDesc& d, long ws_elems) {
   QCA_CHECK(d.M >= 0 && d.N >= 0 && d.K >= 0, "negative GEMM extent");
   QCA_CHECK(d.K2 >= 1 && (d.K == 0 || d.K % d.K2 == 0), "K2 must divide K");
   QCA_CHECK(d.NB1 >= 1 && d.NB2 >= 1, "empty batch");
+  if (d.seg_rows > 0) {
+    QCA_CHECK(d.NB1 == 1 && d.seg_rowptr && d.seg_col && d.seg_outv && d.seg_val && !d.conj_a,
+              "gathered GEMM needs one batch level, the sparse operand and no conjugation of A");
+    QCA_CHECK(d.seg_max >= 0 && d.K == d.seg_max * d.K2, "gathered GEMM: K must be seg_max * K2");
+  }
+  // per-thread element offsets within a tile are 32-bit and non-negative
+  QCA_CHECK(d.a_m >= 0 && d.a_k2 >= 0 && d.b_n >= 0 && d.b_k2 >= 0 && d.a_k1 >= 0 && d.b_k1 >= 0,
+            "GEMM strides must be non-negative");
+  QCA_CHECK((BM - 1) * d.a_m + (BK - 1) * d.a_k2 < (1L << 31) && (BN - 1) * d.b_n + (BK - 1) * d.b_k2 < (1L << 31),
+            "GEMM tile spans more than 2^31 elements");
   GemmPlan p;
   p.batch = d.NB1 * d.NB2;
   if (d.M == 0 || d.N == 0) {
@@ -326,22 +406,20 @@ GemmPlan gemm_plan(const GemmDesc& d, long ws_elems) {
   p.tm = ceil_div(d.M, BM);
   p.tn = ceil_div(d.N, BN);
   const long tiles = (long)p.tm * p.tn * p.batch;
+  // K-tiles of the longest batch: segments of K2, each cut into BK tiles
+  const long ktiles = (long)(d.K / d.K2) * ceil_div(d.K2, BK);
+  QCA_CHECK(ktiles < (1L << 30), "GEMM K too long");
   // split K when the output tiles alone cannot occupy the chip
   int splits = 1;
-  if (tiles < 192 && d.K >= 4 * BK) {
-    splits = (int)std::min<long>((256 + tiles - 1) / tiles, d.K / (2 * BK));
+  if (tiles < 192 && ktiles >= 4) {
+    splits = (int)std::min<long>((256 + tiles - 1) / tiles, ktiles / 2);
     while (splits > 1 && (long)splits * p.batch * d.M * d.N > ws_elems) --splits;
     if (splits < 1) splits = 1;
   }
-  int kps = d.K;
+  int kps = (int)ktiles;  // K-tiles per split
   if (splits > 1) {
-    kps = ceil_div(ceil_div(d.K, splits), BK) * BK;
-    splits = ceil_div(d.K, kps);
-  }
-  if (d.K == 0) {
-    // C = beta*C: only zeroing is needed when not accumulating
-    splits = 1;
-    kps = 0;
+    kps = ceil_div(ktiles, splits);
+    splits = ceil_div(ktiles, kps);
   }
   QCA_CHECK((long)tiles * splits < (1L << 31), "GEMM grid too large");
   QCA_CHECK(p.batch * splits <= 65535, "GEMM batch*splits exceeds grid z limit");
@@ -350,7 +428,7 @@ GemmPlan gemm_plan(const GemmDesc& d, long ws_elems) {
   p.am = (d.a_m == 1);
   p.bn = (d.b_n == 1);
   // two K-groups per workgroup when each has at least two K-tiles to take
-  p.kg2 = kps >= 4 * BK;
+  p.kg2 = kps >= 4;
   if (splits > 1) {
     const long total = (long)d.M * d.N * p.batch;
     p.reduce_blocks = (int)std::min<long>(ceil_div(total, 256), 4096);

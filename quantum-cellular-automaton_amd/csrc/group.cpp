@@ -439,8 +439,10 @@ void Group::evolve(std::vector<int> inst, std::vector<int> kind, std::vector<int
       p.g1.b_jstride = n[i];   // the matvec reads basis vector V_j
     }
     g1.push_back(p.g1);
-    wm.push_back(wmix_rec<T>(p.wm.Tin, p.wm.in_stride, p.wm.nxc, p.wm.nin, p.wm.U, p.wm.nq, p.wm.Dout, *W, skip));
-    wm_lds = std::max(wm_lds, wmix_lds_bytes<T>(p.wm.nin, p.wm.nq, p.wm.Dout));
+    if (p.mix) {
+      wm.push_back(wmix_rec<T>(p.wm.Tin, p.wm.in_stride, p.wm.nxc, p.wm.nin, p.wm.U, p.wm.nq, p.wm.Dout, *W, skip));
+      wm_lds = std::max(wm_lds, wmix_lds_bytes<T>(p.wm.nin, p.wm.nq, p.wm.Dout));
+    }
     g3.push_back(p.g3);
     lz.push_back(z);
     lz_item.push_back((int)i);

@@ -97,7 +97,7 @@ struct GemmArgs {
   cplx<T>* C;
   cplx<T>* ws;
   long b_jstride;   // B advances by j * b_jstride elements in launch(s, j)
-  int splits, kps;
+  int splits, kps;  // kps: K-tiles per split
 };
 
 // One GEMM of one instance, with the split-K workspace it would use alone.

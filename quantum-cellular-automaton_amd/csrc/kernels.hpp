@@ -44,8 +44,11 @@ template <typename T>
 struct WSparseDev {
   int* rowptr = nullptr;
   int* col = nullptr;
+  int* outv = nullptr;   // MPO bond index of each nonzero's row (row % group_rows)
   cplx<T>* val = nullptr;
   int nout = 0, nin = 0, nnz = 0;
+  int group_rows = 0;    // rows per output physical index (the output MPO dimension)
+  int group_max = 0;     // most nonzeros in any group of group_rows rows
 };
 
 template <typename T>

@@ -69,6 +69,15 @@ inline void ensure_dynamic_lds(const void* fn, size_t bytes) {
 // with every index split into at most two levels so that contractions over
 // composite indices (e.g. (MPO bond, bond) or (physical, bond)) map onto one
 // GEMM without a transpose copy.  k = k1*K2 + k2.  batch = b1*NB2 + b2.
+//
+// The K range is walked as segments of K2 (k1 = segment).  In gather form
+// (seg_rows > 0) the segments of batch b2 (NB1 == 1) are instead the nonzeros
+// e of rows [b2*seg_rows, (b2+1)*seg_rows) of a sparse matrix (CSR seg_rowptr,
+// seg_col, seg_outv = row % seg_rows, seg_val): segment e contributes
+//   sum_k2 seg_val[e] * A[m*a_m + seg_col[e]*a_k1 + k2*a_k2]
+//                     * B[seg_outv[e]*b_k1 + k2*b_k2 + n*b_n],
+// i.e. opA is a sparse mix of A's slices formed on the fly.  seg_max bounds
+// the segments of any batch (K = seg_max * K2 for planning).
 struct GemmDesc {
   int M, N, K, K2;  // K2 divides K
   int NB1, NB2;     // batch = NB1 * NB2
@@ -79,6 +88,11 @@ struct GemmDesc {
   double alpha_re, alpha_im;
   int accumulate;  // beta = 1 when set, else 0
   const int* skip; // device flag: launch is a no-op when *skip != 0 (may be null)
+  int seg_rows, seg_max;
+  const int* seg_rowptr;
+  const int* seg_col;
+  const int* seg_outv;
+  const void* seg_val;  // cplx<T> of the GEMM's precision
 };
 
 void gemm_c128(const GemmDesc& d, const c128* A, const c128* B, c128* C, hipStream_t s,
