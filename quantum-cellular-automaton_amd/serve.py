@@ -6,13 +6,15 @@ server returns <n_i> every `report_every` steps, as the reference driver
 measures every plotting interval (quantum_game.py:82-119), and the request's
 latency is the time from its arrival to its last returned <n_i>.
 
-Scheduling: admitted requests sit in a round-robin queue; `workers` host
-threads repeatedly take the front request, advance it by one reporting chunk
-on its own engine (its own HIP stream), measure, and requeue it until done.
-New arrivals join the queue while others run (continuous batching), so a short
-request is never stuck behind a long one.  A request is admitted when its
-device footprint fits in the remaining budget; otherwise it waits in arrival
-order.  With the GPU engine factory the footprint is the exact number of
+Two schedulers share the admission rule.  GroupServer (the fast one) sorts
+admitted requests into size lanes and advances each lane's requests together,
+one grouped lockstep time step per tick (qca_group_step), lanes side by side
+on their own threads and streams.  Server keeps admitted requests in a
+round-robin queue; `workers` host threads repeatedly take the front request,
+advance it by one reporting chunk on its own engine (its own HIP stream),
+measure, and requeue it until done.  In both, new arrivals join while others
+run (continuous batching).  A request is admitted when its device footprint
+fits in the remaining budget; otherwise it waits in arrival order.  With the GPU engine factory the footprint is the exact number of
 bytes the engine will allocate (qca_engine_footprint, the constructor's own
 sizing) and the budget is the device's free memory at start-up, less a
 reserve.  The engine factory is injectable, so the scheduler is tested on the
@@ -180,21 +182,31 @@ class Server:
                 "site_updates_per_s": self.site_updates / wall if wall > 0 else 0.0}
 
 
-class GroupServer:
-    """Continuous batching through grouped lockstep steps.
+def size_lane(inst: workloads.Instance) -> tuple:
+    """Lane of a request in GroupServer: requests whose time steps cost about
+    the same share a lockstep group.  A grouped step walks the longest chain
+    of its group (2(N-1) sweep positions) at the widest bond, so chains up to
+    38 cells, longer chains, and bond caps above 128 go to separate lanes."""
+    return (inst.chi_max > 128, inst.ncells > 38)
 
-    A scheduler loop runs one tick per time step: arrivals that came in since
-    the last tick join the pending queue (arrival order), pending requests are
-    admitted while their footprint fits the budget, and every admitted request
-    advances one time step in ONE grouped step of all of them (qca_group_step:
-    each kernel of a sweep position runs once for all instances).  A request
-    reports <n_i> every ``report_every`` steps and after its last one, then
-    leaves the group, which is rebuilt whenever its membership changes.
-    ``group_factory(engines)`` builds the group (default qca_native.Group), so
+
+class GroupServer:
+    """Continuous batching through grouped lockstep steps, one group per lane.
+
+    Admitted requests are sorted into lanes by ``lane_of(inst)`` (default
+    size_lane).  Each lane runs its own tick loop on its own host thread:
+    every tick, the lane's newly admitted requests join it, and all of its
+    requests advance one time step in ONE grouped step (qca_group_step: each
+    kernel of a sweep position runs once for all of them; a group has its own
+    HIP stream, so lanes run side by side on the GPU).  A request reports
+    <n_i> every ``report_every`` steps and after its last one, then leaves;
+    a lane's group is rebuilt whenever its membership changes.  Requests are
+    admitted in arrival order while their footprint fits the budget.
+    ``group_factory(engines)`` builds a group (default qca_native.Group), so
     the scheduler is tested on the host with stand-ins."""
 
     def __init__(self, make_engine, budget_bytes: int | None = None, report_every: int = 10,
-                 precision: int = 1, device: int = 0, group_factory=None):
+                 precision: int = 1, device: int = 0, group_factory=None, lane_of=size_lane):
         self.make_engine = make_engine
         if budget_bytes is None:
             import qca_native as qn
@@ -208,11 +220,13 @@ class GroupServer:
             import qca_native as qn
             group_factory = qn.Group
         self.group_factory = group_factory
+        self.lane_of = lane_of
         self.in_use = 0
         self.peak_in_use = 0
         self.peak_active = 0
         self.group_builds = 0
         self.ticks = 0
+        self.lane_ticks: dict = {}
         self.finished: list = []
         self.site_updates = 0
 
@@ -220,9 +234,64 @@ class GroupServer:
         return (self._footprint(req.inst) if self._footprint is not None
                 else footprint_bytes(req.inst, self.precision))
 
+    def _lane(self, key, inbox: collections.deque, cond: threading.Condition, state: dict):
+        """Tick loop of one lane (runs on its own thread)."""
+        active: list = []
+        group, members = None, ()
+        try:
+            while True:
+                with cond:
+                    while not inbox and not active and not state["stop"]:
+                        cond.wait(0.05)
+                    if state["error"] is not None or (not inbox and not active and state["stop"]):
+                        return
+                    active.extend(inbox)
+                    inbox.clear()
+                key_m = tuple(r.rid for r in active)
+                if key_m != members:
+                    if group is not None:
+                        group.close()
+                    group = self.group_factory([r.engine for r in active])
+                    members = key_m
+                    with cond:
+                        self.group_builds += 1
+                group.step(1)
+                done, still = [], []
+                for r in active:
+                    r.done_steps += 1
+                    last = r.done_steps >= r.steps
+                    if last or r.done_steps % self.report_every == 0:
+                        r.reports.append(np.asarray(r.engine.rho()[:, 1, 1].real))
+                    (done if last else still).append(r)
+                if done:
+                    group.close()
+                    group, members = None, ()
+                    for r in done:
+                        r.engine.close()
+                        r.engine = None
+                        r.finished = time.perf_counter()
+                with cond:
+                    self.ticks += 1
+                    self.lane_ticks[key] = self.lane_ticks.get(key, 0) + 1
+                    self.site_updates += sum(r.inst.site_updates_per_step for r in active)
+                    for r in done:
+                        self.in_use -= r.footprint
+                        self.finished.append(r)
+                    state["active"] -= len(done)
+                    cond.notify_all()
+                active = still
+        except BaseException as exc:  # handed to the scheduler, which stops every lane
+            with cond:
+                state["error"] = state["error"] or exc
+                state["stop"] = True
+                cond.notify_all()
+        finally:
+            if group is not None:
+                group.close()
+
     def run(self, arrivals: list) -> dict:
         arrived = collections.deque()
-        lock = threading.Lock()
+        feed_lock = threading.Lock()
         t0 = time.perf_counter()
         feeding = {"done": False}
 
@@ -232,72 +301,61 @@ class GroupServer:
                 if delay > 0:
                     time.sleep(delay)
                 req.arrival = time.perf_counter()
-                with lock:
+                with feed_lock:
                     arrived.append(req)
             feeding["done"] = True
 
         feeder = threading.Thread(target=feed, daemon=True)
         feeder.start()
+        cond = threading.Condition()
+        state = {"stop": False, "error": None, "active": 0}
+        lanes: dict = {}
         pending = collections.deque()
-        active: list = []
-        group, members = None, ()
         try:
             while True:
-                with lock:
+                with feed_lock:
                     while arrived:
                         req = arrived.popleft()
                         req.footprint = self._size(req)
                         if req.footprint > self.budget:
                             raise ValueError(f"request {req.rid} needs {req.footprint} B, above the budget")
                         pending.append(req)
-                while pending and self.in_use + pending[0].footprint <= self.budget:
-                    req = pending.popleft()
+                    fed = feeding["done"] and not arrived
+                admitted = []
+                with cond:
+                    if state["error"] is not None:
+                        break
+                    while pending and self.in_use + pending[0].footprint <= self.budget:
+                        req = pending.popleft()
+                        self.in_use += req.footprint
+                        self.peak_in_use = max(self.peak_in_use, self.in_use)
+                        state["active"] += 1
+                        admitted.append(req)
+                    self.peak_active = max(self.peak_active, state["active"])
+                for req in admitted:
                     req.engine = self.make_engine(req.inst)
-                    self.in_use += req.footprint
-                    self.peak_in_use = max(self.peak_in_use, self.in_use)
-                    active.append(req)
-                if not active:
-                    if feeding["done"] and not pending:
-                        with lock:
-                            if not arrived:
-                                break
-                    time.sleep(0.001)
-                    continue
-                key = tuple(r.rid for r in active)
-                if key != members:
-                    if group is not None:
-                        group.close()
-                    group = self.group_factory([r.engine for r in active])
-                    members = key
-                    self.group_builds += 1
-                self.peak_active = max(self.peak_active, len(active))
-                group.step(1)
-                self.ticks += 1
-                still = []
-                for r in active:
-                    r.done_steps += 1
-                    self.site_updates += r.inst.site_updates_per_step
-                    last = r.done_steps >= r.steps
-                    if last or r.done_steps % self.report_every == 0:
-                        r.reports.append(np.asarray(r.engine.rho()[:, 1, 1].real))
-                    if last:
-                        r.finished = time.perf_counter()
-                        self.finished.append(r)
-                    else:
-                        still.append(r)
-                if len(still) != len(active):
-                    group.close()
-                    group, members = None, ()
-                    for r in active:
-                        if r.finished is not None:
-                            r.engine.close()
-                            r.engine = None
-                            self.in_use -= r.footprint
-                active = still
+                    key = self.lane_of(req.inst)
+                    if key not in lanes:
+                        inbox = collections.deque()
+                        th = threading.Thread(target=self._lane, args=(key, inbox, cond, state), daemon=True)
+                        lanes[key] = (inbox, th)
+                        th.start()
+                    with cond:
+                        lanes[key][0].append(req)
+                        cond.notify_all()
+                with cond:
+                    if fed and not pending and state["active"] == 0:
+                        break
+                    cond.wait(0.002)
         finally:
-            if group is not None:
-                group.close()
+            with cond:
+                state["stop"] = True
+                cond.notify_all()
+            for _, th in lanes.values():
+                th.join()
             feeder.join()
+        if state["error"] is not None:
+            raise state["error"]
         wall = time.perf_counter() - t0
         lat = np.array([r.finished - r.arrival for r in self.finished])
         return {"requests": len(self.finished), "wall_s": wall,
@@ -305,9 +363,11 @@ class GroupServer:
                 "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
                 "site_updates": self.site_updates,
                 "site_updates_per_s": self.site_updates / wall if wall > 0 else 0.0,
-                "ticks": self.ticks, "group_builds": self.group_builds, "peak_active": self.peak_active,
-                "peak_footprint_bytes": self.peak_in_use, "budget_bytes": self.budget,
-                "method": "grouped lockstep ticks (qca_group_step), one time step per tick"}
+                "ticks": self.ticks, "lanes": len(lanes), "group_builds": self.group_builds,
+                "peak_active": self.peak_active, "peak_footprint_bytes": self.peak_in_use,
+                "budget_bytes": self.budget,
+                "method": "grouped lockstep ticks (qca_group_step), one group per size lane, lanes on "
+                          "their own threads and streams"}
 
 
 def poisson_arrivals(rate: float, window_s: float, seed: int = 5, big_fraction: float = 0.1,

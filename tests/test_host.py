@@ -297,9 +297,10 @@ def test_engine_footprint_is_host_computed_and_ordered():
 
 def test_group_server_schedules_every_request():
     """Grouped continuous batching on the host (stand-in engines and group):
-    every request advances one step per tick while admitted, reports every 10
-    steps and after its last step, admission keeps the footprint within the
-    budget, and the group is rebuilt when its membership changes."""
+    every request advances one step per tick of its lane while admitted,
+    reports every 10 steps and after its last step, admission keeps the
+    footprint within the budget, and a lane's group is rebuilt when its
+    membership changes."""
     import threading
     import serve
     _FakeEngine.lock = threading.Lock()
@@ -331,3 +332,6 @@ def test_group_server_schedules_every_request():
     assert out["site_updates"] == sum(r.steps * r.inst.site_updates_per_step for _, r in arrivals)
     assert _FakeEngine.live == 0 and srv.in_use == 0 and 0 < srv.peak_in_use <= budget
     assert out["group_builds"] == len(FakeGroup.built) >= 2 and max(FakeGroup.built) >= 2
+    # the draws span several size lanes, each stepped as its own group
+    lanes = {serve.size_lane(r.inst) for _, r in arrivals}
+    assert out["lanes"] == len(lanes) >= 2
