@@ -5,11 +5,13 @@
 // Gram matrix.  The columns are cut into blocks of bs <= 16; the blocks meet in
 // the round-robin tournament, one outer round per kernel triple:
 //   jac_gram   G_p = X_p^H X_p for every block pair p (2bs columns), each
-//              workgroup one chunk of 64 rows, partial Grams to a workspace;
+//              workgroup one chunk of 64 rows on the FP64 matrix cores,
+//              partial Grams to a workspace;
 //   jac_inner  one workgroup per pair sums its partial Grams and runs one
 //              cyclic sweep of two-sided Jacobi rotations on G_p in LDS
 //              (G <- R^H G R), accumulating J_p = product of the rotations;
-//   jac_apply  X_p <- X_p J_p, each workgroup one chunk of rows.
+//   jac_apply  X_p <- X_p J_p, each workgroup one chunk of rows, on the
+//              FP64 matrix cores.
 // Stream order separates the rounds.  The Gram is rebuilt from the columns in
 // every round, so the test for a converged pair, |x_i^H x_j| <=
 // tol*|x_i||x_j|, and the singular values (final column norms) carry the
@@ -39,6 +41,8 @@ constexpr int JAC_MAXCOLS = 512;
 constexpr int JAC_BS = 16;           // block size: pairs of 32 columns
 constexpr int JAC_LMAX = 2 * JAC_BS;
 constexpr int JAC_RC = 64;           // rows per Gram / apply chunk
+constexpr int JAC_GRAM_NT = 256;     // jac_gram: one wave per 16x16 block of the 32x32 Gram
+constexpr int JAC_APPLY_NT = 512;    // jac_apply: one wave per 16x16 block of the 32x64 product
 constexpr int JAC_MAX_SWEEPS = 60;
 // Noise floor of a column, squared and relative to ||X||_F^2: (16 eps)^2.
 // Columns below 16 eps ||X|| (3.6e-15 ||X||) are not orthogonalised further;
@@ -69,20 +73,31 @@ struct JacobiGeom {
 };
 
 // Stages rows [r0, r0+nr) of the 2bs columns of pair p into a row-major LDS
-// tile with a padded row stride (JAC_LMAX + 1) that spreads the banks.  Lane
-// bits give the row, the wave index walks the columns: no divisions, and each
-// wave reads one contiguous 64-row run of a column.
+// tile with a padded row stride (JAC_LMAX + 1) that spreads the banks; rows
+// beyond nr and columns beyond 2bs (up to JAC_LMAX) are zero.  Lane bits give
+// the row, the wave index walks the columns: no divisions, and each wave reads
+// one contiguous 64-row run of a column.
 __device__ void stage_tile(const c128* __restrict__ X, const JacobiGeom& g, int p, int round,
                            int r0, int nr, c128* tile) {
   static_assert(JAC_RC == 64, "one row per lane");
   const int L = 2 * g.bs;
   const int bi = rr_player(p, round, g.nb), bj = rr_player(g.nb - 1 - p, round, g.nb);
   const int r = threadIdx.x & 63;
-  for (int c = threadIdx.x >> 6; c < L; c += JAC_NT / 64) {
+  for (int c = threadIdx.x >> 6; c < JAC_LMAX; c += blockDim.x / 64) {
     const int gc = c < g.bs ? bi * g.bs + c : bj * g.bs + (c - g.bs);
     tile[r * (JAC_LMAX + 1) + c] =
-        (gc < g.cols && r < nr) ? X[(long)gc * g.rows + r0 + r] : c128{0.0, 0.0};
+        (c < L && gc < g.cols && r < nr) ? X[(long)gc * g.rows + r0 + r] : c128{0.0, 0.0};
   }
+}
+
+// The pair products of a round on the FP64 matrix cores: v_mfma_f64_16x16x4f64
+// takes A[m][k] from lane m + 16k and B[k][n] from lane n + 16k, and register
+// r of lane l holds D[(l / 16) + 4r][l % 16].  A complex product is four real
+// MFMAs.  One wave per 16x16 output tile, the whole K range in that wave, in
+// order: the sums are deterministic.
+typedef double jac_f64x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ jac_f64x4 jac_mfma(double a, double b, jac_f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
 // ||X||_F^2 and reset of the control block (one workgroup).
@@ -110,6 +125,8 @@ __global__ __launch_bounds__(JAC_NT) void jac_init(const c128* __restrict__ X, l
 
 // Partial Gram of pair blockIdx.x over row chunk blockIdx.y:
 // Gpart[p][kc](i, j) = sum_r conj(X(r, c_i)) X(r, c_j), stored column-major.
+// Four waves, wave w computing the 16x16 block (j in 16(w/2).., i in 16(w%2)..)
+// as D[j][i] = sum_r X(r, c_j) conj(X(r, c_i)) over the chunk's 64 rows.
 __device__ __forceinline__ void jac_gram_body(const c128* __restrict__ X, const JacobiGeom& g, int round,
                                               c128* __restrict__ Gpart,
                                               const JacobiCtrl* __restrict__ ctrl, int p, int kc) {
@@ -119,19 +136,28 @@ __device__ __forceinline__ void jac_gram_body(const c128* __restrict__ X, const 
   const int r0 = kc * JAC_RC, nr = min(JAC_RC, g.rows - r0);
   stage_tile(X, g, p, round, r0, nr, tile);
   __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int tj = (wave >> 1) * 16, ti = (wave & 1) * 16;
+  if (tj >= L || ti >= L) return;
+  const int q = lane >> 4, c = lane & 15;
+  jac_f64x4 gre = {0, 0, 0, 0}, gim = {0, 0, 0, 0};
+  const int kmax = (nr + 3) & ~3;
+  for (int k0 = 0; k0 < kmax; k0 += 4) {
+    const c128 a = tile[(k0 + q) * (JAC_LMAX + 1) + tj + c];   // X(r, c_j)
+    const c128 b = tile[(k0 + q) * (JAC_LMAX + 1) + ti + c];   // X(r, c_i), conjugated below
+    gre = jac_mfma(a.re, b.re, gre);
+    gre = jac_mfma(a.im, b.im, gre);
+    gim = jac_mfma(a.im, b.re, gim);
+    gim = jac_mfma(-a.re, b.im, gim);
+  }
   c128* out = Gpart + ((long)p * g.kspl + kc) * L * L;
-  for (int e = threadIdx.x; e < L * L; e += JAC_NT) {
-    const int i = e % L, j = e / L;
-    double re = 0, im = 0;
-    for (int r = 0; r < nr; ++r) {
-      const c128 u = tile[r * (JAC_LMAX + 1) + i], v = tile[r * (JAC_LMAX + 1) + j];
-      re += u.re * v.re + u.im * v.im;
-      im += u.re * v.im - u.im * v.re;
-    }
-    out[e] = c128{re, im};
+#pragma unroll
+  for (int rr = 0; rr < 4; ++rr) {
+    const int j = tj + q + 4 * rr, i = ti + c;
+    if (i < L && j < L) out[i + j * L] = c128{gre[rr], gim[rr]};
   }
 }
-__global__ __launch_bounds__(JAC_NT) void jac_gram(const c128* __restrict__ X, JacobiGeom g,
+__global__ __launch_bounds__(JAC_GRAM_NT) void jac_gram(const c128* __restrict__ X, JacobiGeom g,
                                                    int round, c128* __restrict__ Gpart,
                                                    const JacobiCtrl* __restrict__ ctrl) {
   jac_gram_body(X, g, round, Gpart, ctrl, blockIdx.x, blockIdx.y);
@@ -292,29 +318,44 @@ __device__ __forceinline__ void jac_apply_body(c128* __restrict__ Xa, const Jaco
   const JacobiGeom g = second ? gz : ga;
   const int kc = second ? by - ga.kspl : by;
   __shared__ c128 tile[JAC_RC * (JAC_LMAX + 1)];
-  __shared__ c128 J[JAC_LMAX * JAC_LMAX];
+  __shared__ c128 J[JAC_LMAX * (JAC_LMAX + 1)];   // J(k, c) at c * (JAC_LMAX + 1) + k, zero-padded
   const int L = 2 * g.bs;
   const int r0 = kc * JAC_RC, nr = min(JAC_RC, g.rows - r0);
   stage_tile(X, g, p, round, r0, nr, tile);
   const c128* jp = Jin + (long)p * L * L;
-  for (int e = threadIdx.x; e < L * L; e += JAC_NT) J[e] = jp[e];
+  for (int e = threadIdx.x; e < JAC_LMAX * JAC_LMAX; e += blockDim.x) {
+    const int k = e % JAC_LMAX, c = e / JAC_LMAX;
+    J[c * (JAC_LMAX + 1) + k] = (k < L && c < L) ? jp[c * L + k] : c128{0.0, 0.0};
+  }
   __syncthreads();
+  // Eight waves: wave w computes the 16x16 block (columns c in 16(w%2)..,
+  // rows r in 16(w/2)..) of the transposed product D[c][r] = sum_k J(k, c) X(r, k),
+  // so that consecutive lanes store consecutive rows of a column.
   const int bi = rr_player(p, round, g.nb), bj = rr_player(g.nb - 1 - p, round, g.nb);
-  const int r = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int tc = (wave & 1) * 16, tr = (wave >> 1) * 16;
+  if (tc >= L || tr >= nr) return;
+  const int q = lane >> 4, cl = lane & 15;
+  jac_f64x4 dre = {0, 0, 0, 0}, dim = {0, 0, 0, 0};
+  const int kmax = (L + 3) & ~3;
+  for (int k0 = 0; k0 < kmax; k0 += 4) {
+    const c128 a = J[(tc + cl) * (JAC_LMAX + 1) + k0 + q];      // J(k, c)
+    const c128 b = tile[(tr + cl) * (JAC_LMAX + 1) + k0 + q];   // X(r, k)
+    dre = jac_mfma(a.re, b.re, dre);
+    dre = jac_mfma(-a.im, b.im, dre);
+    dim = jac_mfma(a.re, b.im, dim);
+    dim = jac_mfma(a.im, b.re, dim);
+  }
+  const int r = tr + cl;
   if (r < nr)
-    for (int c = threadIdx.x >> 6; c < L; c += JAC_NT / 64) {
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int c = tc + q + 4 * rr;
       const int gc = c < g.bs ? bi * g.bs + c : bj * g.bs + (c - g.bs);
-      if (gc >= g.cols) continue;
-      double re = 0, im = 0;
-      for (int k = 0; k < L; ++k) {
-        const c128 x = tile[r * (JAC_LMAX + 1) + k], j = J[c * L + k];
-        re += x.re * j.re - x.im * j.im;
-        im += x.re * j.im + x.im * j.re;
-      }
-      X[(long)gc * g.rows + r0 + r] = c128{re, im};
+      if (c < L && gc < g.cols) X[(long)gc * g.rows + r0 + r] = c128{dre[rr], dim[rr]};
     }
 }
-__global__ __launch_bounds__(JAC_NT) void jac_apply(c128* __restrict__ Xa, JacobiGeom ga,
+__global__ __launch_bounds__(JAC_APPLY_NT) void jac_apply(c128* __restrict__ Xa, JacobiGeom ga,
                                                     c128* __restrict__ Za, JacobiGeom gz, int round,
                                                     const c128* __restrict__ Jin,
                                                     const int* __restrict__ rotated,
@@ -958,7 +999,7 @@ __global__ __launch_bounds__(JAC_NT) void jac_init_grouped(const GRec<JacArgs>* 
   jac_init_body(a.Y, (long)a.g.rows * a.g.cols, a.ctrl);
 }
 // Rounds beyond a matrix's own tournament (nb - 1 rounds) are no-ops for it.
-__global__ __launch_bounds__(JAC_NT) void jac_gram_grouped(const GRec<JacArgs>* __restrict__ recs, int n,
+__global__ __launch_bounds__(JAC_GRAM_NT) void jac_gram_grouped(const GRec<JacArgs>* __restrict__ recs, int n,
                                                            int round) {
   unsigned bx, by, bz;
   const GRec<JacArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
@@ -974,7 +1015,7 @@ __global__ __launch_bounds__(JAC_NT) void jac_inner_grouped(const GRec<JacArgs>*
   if (round >= g.nb - 1) return;
   jac_inner_body(g, sweep, r.a.Gpart, r.a.Jbuf, r.a.rotated, r.a.tol, r.a.ctrl, bx);
 }
-__global__ __launch_bounds__(JAC_NT) void jac_apply_grouped(const GRec<JacArgs>* __restrict__ recs, int n,
+__global__ __launch_bounds__(JAC_APPLY_NT) void jac_apply_grouped(const GRec<JacArgs>* __restrict__ recs, int n,
                                                             int round) {
   unsigned bx, by, bz;
   const GRec<JacArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
@@ -1048,11 +1089,11 @@ JacobiCtrl* gram_jacobi(c128* Y, int rows, int cols, c128* Z, int zrows, hipStre
   while (sweep < JAC_MAX_SWEEPS) {
     for (int c = 0; c < chunk && sweep < JAC_MAX_SWEEPS; ++c, ++sweep) {
       for (int r = 0; r < l.nb - 1; ++r) {
-        hipLaunchKernelGGL(jac_gram, dim3(np, l.kspl), dim3(JAC_NT), 0, s, (const c128*)Y, g, r,
+        hipLaunchKernelGGL(jac_gram, dim3(np, l.kspl), dim3(JAC_GRAM_NT), 0, s, (const c128*)Y, g, r,
                            Gpart, (const JacobiCtrl*)ctrl);
         hipLaunchKernelGGL(jac_inner, dim3(np), dim3(JAC_NT), 0, s, g, sweep, (const c128*)Gpart,
                            Jbuf, rotated, tol, ctrl);
-        hipLaunchKernelGGL(jac_apply, dim3(np, l.kspl + gz.kspl), dim3(JAC_NT), 0, s, Y, g, Z, gz, r,
+        hipLaunchKernelGGL(jac_apply, dim3(np, l.kspl + gz.kspl), dim3(JAC_APPLY_NT), 0, s, Y, g, Z, gz, r,
                            (const c128*)Jbuf, (const int*)rotated, (const JacobiCtrl*)ctrl);
       }
       hipLaunchKernelGGL(jac_check, dim3(1), dim3(64), 0, s, ctrl, sweep);
@@ -1395,11 +1436,11 @@ void grouped_jacobi_svd(const std::vector<SvdJob>& jobs, RecordArena& arena, Gro
     while (sweep < JAC_MAX_SWEEPS) {
       for (int c = 0; c < chunk && sweep < JAC_MAX_SWEEPS; ++c, ++sweep) {
         for (int r = 0; r < max_rounds; ++r) {
-          hipLaunchKernelGGL(jac_gram_grouped, dim3(g_gram.l.blocks), dim3(JAC_NT), 0, s, g_gram.l.recs,
+          hipLaunchKernelGGL(jac_gram_grouped, dim3(g_gram.l.blocks), dim3(JAC_GRAM_NT), 0, s, g_gram.l.recs,
                              g_gram.l.n, r);
           hipLaunchKernelGGL(jac_inner_grouped, dim3(g_inner.l.blocks), dim3(JAC_NT), 0, s, g_inner.l.recs,
                              g_inner.l.n, r, sweep);
-          hipLaunchKernelGGL(jac_apply_grouped, dim3(g_apply.l.blocks), dim3(JAC_NT), 0, s, g_apply.l.recs,
+          hipLaunchKernelGGL(jac_apply_grouped, dim3(g_apply.l.blocks), dim3(JAC_APPLY_NT), 0, s, g_apply.l.recs,
                              g_apply.l.n, r);
         }
         hipLaunchKernelGGL(jac_check_grouped, dim3(nj), dim3(64), 0, s, g_one.l.recs, nj, sweep);
