@@ -14,6 +14,8 @@
 // operand loads (GemmDesc's gather form) instead of a separate pass.
 // Launches with too few output tiles to fill 256 CUs split K and reduce the
 // partial tiles in a second, deterministic pass.
+#include <cstdlib>
+
 #include "grouped.hpp"
 
 namespace qca {
@@ -79,11 +81,12 @@ __device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
 // registers during iteration i-1 is stored into stage i%2 while stage
 // (i-1)%2 may still be read, so one barrier per K-tile suffices, and the
 // global loads of tile i+1 are in flight during the MFMAs of tile i.
-template <typename T, bool A_MCONT, bool B_NCONT, int KG>
+template <typename T, bool A_MCONT, bool B_NCONT, int KG, bool FUSED>
 __device__ __forceinline__ void gemm_body(const GemmDesc& d, const cplx<T>* __restrict__ A,
                                           const cplx<T>* __restrict__ B, cplx<T>* __restrict__ C,
                                           cplx<T>* __restrict__ ws, int splits, int tiles_per_split,
                                           unsigned bx, unsigned by, unsigned bz) {
+  constexpr bool fused = FUSED;
   using acc_t = typename Mfma<T>::acc_t;
   using PA = Plane<T, A_MCONT>;
   using PB = Plane<T, B_NCONT>;
@@ -102,8 +105,11 @@ __device__ __forceinline__ void gemm_body(const GemmDesc& d, const cplx<T>* __re
   const int wn = (wave & 1) * 32;
   const int m0 = by * BM;
   const int n0 = bx * BN;
-  const int zb = bz / splits;
-  const int split = bz - zb * splits;
+  // fused: this workgroup takes every split of its output tile in turn (bz is
+  // the batch) and sums them in split order itself; otherwise bz = batch *
+  // splits + split and the partial tile goes to the split-K workspace
+  const int zb = fused ? (int)bz : (int)bz / splits;
+  const int split = fused ? 0 : (int)bz - zb * splits;
   const int b1 = zb / d.NB2;
   const int b2 = zb - b1 * d.NB2;
 
@@ -119,9 +125,18 @@ __device__ __forceinline__ void gemm_body(const GemmDesc& d, const cplx<T>* __re
     nseg = d.K / d.K2;
   }
   const int tps = (d.K2 + BK - 1) / BK;
-  const int t_begin = split * tiles_per_split;
-  const int t_end = min(nseg * tps, t_begin + tiles_per_split);
-  const int niter = t_end > t_begin ? (t_end - t_begin + KG - 1) / KG : 0;
+  const int ntiles = nseg * tps;
+  const int ch_begin = fused ? 0 : split, ch_end = fused ? splits : split + 1;
+  // K-group iterations of split (chunk) ch, and the tile of iteration li (-1: none)
+  auto chunk_iters = [&](int ch) {
+    const int tb = ch * tiles_per_split, te = min(ntiles, tb + tiles_per_split);
+    return te > tb ? (te - tb + KG - 1) / KG : 0;
+  };
+  auto tile_at = [&](int ch, int li) {
+    const int tb = ch * tiles_per_split, te = min(ntiles, tb + tiles_per_split);
+    const int t = tb + li * KG + grp;
+    return t < te ? t : -1;
+  };
 
   // element offsets of this thread's loads within a tile (strides are
   // non-negative and a tile spans less than 2^31 elements: checked on the host)
@@ -160,7 +175,7 @@ __device__ __forceinline__ void gemm_body(const GemmDesc& d, const cplx<T>* __re
   cplx<T> pa[LOADS], pb[LOADS];
   cplx<T> wgt{T(1), T(0)};
   auto load_tile = [&](int t) {
-    if (t >= t_end) {  // uniform per K-group: no memory access
+    if (t < 0) {  // uniform per K-group: no memory access
 #pragma unroll
       for (int r = 0; r < LOADS; ++r) pa[r] = pb[r] = cplx<T>{T(0), T(0)};
       return;
@@ -198,100 +213,143 @@ __device__ __forceinline__ void gemm_body(const GemmDesc& d, const cplx<T>* __re
 
   auto stage_ptr = [&](int stage) { return smem + (grp * 2 + stage) * STAGE; };
 
-  load_tile(t_begin + grp);
-  for (int it = 0; it < niter; ++it) {
-    const int t = t_begin + it * KG + grp;
-    T* st = stage_ptr(it & 1);
-    T* a_re = st;
-    T* a_im = st + PA::SIZE;
-    T* b_re = st + 2 * PA::SIZE;
-    T* b_im = st + 2 * PA::SIZE + PB::SIZE;
-    // ---- registers -> LDS (re/im planes), then start this group's next tile
-    const cplx<T> w = wgt;
+  acc_t tot_re[2][2], tot_im[2][2];   // fused: sum of the finished splits
 #pragma unroll
-    for (int r = 0; r < LOADS; ++r) {
-      const int e = tid + 256 * r;
-      const int ml = A_MCONT ? (e % BM) : (e / BK);
-      const int kl = A_MCONT ? (e / BM) : (e % BK);
-      const cplx<T> v = gather ? cmul(pa[r], w) : pa[r];
-      a_re[PA::at(kl, ml)] = v.re;
-      a_im[PA::at(kl, ml)] = sa * v.im;
-    }
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int r = 0; r < LOADS; ++r) {
-      const int e = tid + 256 * r;
-      const int nl = B_NCONT ? (e % BN) : (e / BK);
-      const int kl = B_NCONT ? (e / BN) : (e % BK);
-      b_re[PB::at(kl, nl)] = pb[r].re;
-      b_im[PB::at(kl, nl)] = sb * pb[r].im;
+    for (int j = 0; j < 2; ++j) {
+      tot_re[i][j] = acc_t{0, 0, 0, 0};
+      tot_im[i][j] = acc_t{0, 0, 0, 0};
     }
-    load_tile(t + KG);
-    __syncthreads();
+  T* const xs = smem + (wave * 32) * 64 + lane;  // partial-tile exchange: [wave][value][lane]
 
-    // a group without a tile in the last round skips its MFMAs (the barrier
-    // stays outside)
-    if (t < t_end) {
-      const int c = lane & 15;
-      const int q = lane >> 4;
-      T ar[4][2], ai[4][2], br[4][2], bi[4][2];
+  load_tile(ch_begin < ch_end ? tile_at(ch_begin, 0) : -1);
+  int it = 0;  // iterations so far: the LDS stage parity
+  for (int ch = ch_begin; ch < ch_end; ++ch) {
+    const int nit = chunk_iters(ch);
+    for (int li = 0; li < nit; ++li, ++it) {
+      const int t = tile_at(ch, li);
+      T* st = stage_ptr(it & 1);
+      T* a_re = st;
+      T* a_im = st + PA::SIZE;
+      T* b_re = st + 2 * PA::SIZE;
+      T* b_im = st + 2 * PA::SIZE + PB::SIZE;
+      // ---- registers -> LDS (re/im planes), then start this group's next tile
+      const cplx<T> w = wgt;
 #pragma unroll
-      for (int kk = 0; kk < 4; ++kk)
+      for (int r = 0; r < LOADS; ++r) {
+        const int e = tid + 256 * r;
+        const int ml = A_MCONT ? (e % BM) : (e / BK);
+        const int kl = A_MCONT ? (e / BM) : (e % BK);
+        const cplx<T> v = gather ? cmul(pa[r], w) : pa[r];
+        a_re[PA::at(kl, ml)] = v.re;
+        a_im[PA::at(kl, ml)] = sa * v.im;
+      }
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-          ar[kk][i] = a_re[PA::at(4 * kk + q, wm + 16 * i + c)];
-          ai[kk][i] = a_im[PA::at(4 * kk + q, wm + 16 * i + c)];
-          br[kk][i] = b_re[PB::at(4 * kk + q, wn + 16 * i + c)];
-          bi[kk][i] = b_im[PB::at(4 * kk + q, wn + 16 * i + c)];
-        }
-#pragma unroll
-      for (int kk = 0; kk < 4; ++kk) {
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int j = 0; j < 2; ++j) {
-            acc_re[i][j] = Mfma<T>::mac(ar[kk][i], br[kk][j], acc_re[i][j]);
-            acc_im[i][j] = Mfma<T>::mac(ar[kk][i], bi[kk][j], acc_im[i][j]);
-          }
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int j = 0; j < 2; ++j) {
-            acc_re[i][j] = Mfma<T>::mac(-ai[kk][i], bi[kk][j], acc_re[i][j]);
-            acc_im[i][j] = Mfma<T>::mac(ai[kk][i], br[kk][j], acc_im[i][j]);
+      for (int r = 0; r < LOADS; ++r) {
+        const int e = tid + 256 * r;
+        const int nl = B_NCONT ? (e % BN) : (e / BK);
+        const int kl = B_NCONT ? (e / BN) : (e % BK);
+        b_re[PB::at(kl, nl)] = pb[r].re;
+        b_im[PB::at(kl, nl)] = sb * pb[r].im;
+      }
+      // the next tile of this K-group: this split's next iteration, else the
+      // first iteration of the next split that has tiles
+      int tn = -1;
+      if (li + 1 < nit) {
+        tn = tile_at(ch, li + 1);
+      } else {
+        for (int c2 = ch + 1; c2 < ch_end; ++c2)
+          if (chunk_iters(c2) > 0) {
+            tn = tile_at(c2, 0);
+            break;
           }
       }
+      load_tile(tn);
+      __syncthreads();
+
+      // a group without a tile in the last round skips its MFMAs (the barrier
+      // stays outside)
+      if (t >= 0) {
+        const int c = lane & 15;
+        const int q = lane >> 4;
+        T ar[4][2], ai[4][2], br[4][2], bi[4][2];
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+          for (int i = 0; i < 2; ++i) {
+            ar[kk][i] = a_re[PA::at(4 * kk + q, wm + 16 * i + c)];
+            ai[kk][i] = a_im[PA::at(4 * kk + q, wm + 16 * i + c)];
+            br[kk][i] = b_re[PB::at(4 * kk + q, wn + 16 * i + c)];
+            bi[kk][i] = b_im[PB::at(4 * kk + q, wn + 16 * i + c)];
+          }
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+              acc_re[i][j] = Mfma<T>::mac(ar[kk][i], br[kk][j], acc_re[i][j]);
+              acc_im[i][j] = Mfma<T>::mac(ar[kk][i], bi[kk][j], acc_im[i][j]);
+            }
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+              acc_re[i][j] = Mfma<T>::mac(-ai[kk][i], bi[kk][j], acc_re[i][j]);
+              acc_im[i][j] = Mfma<T>::mac(ai[kk][i], br[kk][j], acc_im[i][j]);
+            }
+        }
+      }
+    }
+    // end of the split: with KG = 2 group 1 hands its partial tile to group 0
+    // through the operand LDS (once every wave is done reading it; the last
+    // barrier keeps the next tile's stores off the exchange area until group 0
+    // has read it)
+    if constexpr (KG == 2) {
+      __syncthreads();
+      if (grp == 1) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              xs[(((i * 2 + j) * 4 + r) * 2 + 0) * 64] = acc_re[i][j][r];
+              xs[(((i * 2 + j) * 4 + r) * 2 + 1) * 64] = acc_im[i][j][r];
+            }
+      }
+      __syncthreads();
+      if (grp == 0) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              acc_re[i][j][r] += xs[(((i * 2 + j) * 4 + r) * 2 + 0) * 64];
+              acc_im[i][j][r] += xs[(((i * 2 + j) * 4 + r) * 2 + 1) * 64];
+            }
+      }
+      __syncthreads();
+    }
+    if (fused) {
+      // the split's partial tile joins the sum in split order, as splitk_reduce adds it
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            tot_re[i][j][r] += acc_re[i][j][r];
+            tot_im[i][j][r] += acc_im[i][j][r];
+          }
+          acc_re[i][j] = acc_t{0, 0, 0, 0};
+          acc_im[i][j] = acc_t{0, 0, 0, 0};
+        }
     }
   }
 
-  // group 0 writes the tile; with KG = 2 group 1 first hands it its partial
-  // tile through the operand LDS, once every wave is done reading it
-  if constexpr (KG == 2) {
-    __syncthreads();
-    T* xs = smem + (wave * 32) * 64 + lane;  // [wave][value][lane]
-    if (grp == 1) {
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            xs[(((i * 2 + j) * 4 + r) * 2 + 0) * 64] = acc_re[i][j][r];
-            xs[(((i * 2 + j) * 4 + r) * 2 + 1) * 64] = acc_im[i][j][r];
-          }
-    }
-    __syncthreads();
-    if (grp == 0) {
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            acc_re[i][j][r] += xs[(((i * 2 + j) * 4 + r) * 2 + 0) * 64];
-            acc_im[i][j][r] += xs[(((i * 2 + j) * 4 + r) * 2 + 1) * 64];
-          }
-    }
-  }
   const T al_re = T(d.alpha_re), al_im = T(d.alpha_im);
   auto store_out = [&](int m, int n, T xr, T xi) {
     cplx<T>* cp = C + b1 * d.c_b1 + b2 * d.c_b2 + m * d.c_m + n * d.c_n;
@@ -314,9 +372,13 @@ __device__ __forceinline__ void gemm_body(const GemmDesc& d, const cplx<T>* __re
           const int m = m0 + wm + 16 * i + Mfma<T>::row(lane, r);
           const int n = n0 + wn + 16 * j + col;
           if (m >= d.M || n >= d.N) continue;
-          const T xr = acc_re[i][j][r], xi = acc_im[i][j][r];
-          if (splits > 1) ws[((long)bz * d.M + m) * d.N + n] = cplx<T>{xr, xi};
-          else store_out(m, n, xr, xi);
+          if (fused) {
+            store_out(m, n, tot_re[i][j][r], tot_im[i][j][r]);
+          } else {
+            const T xr = acc_re[i][j][r], xi = acc_im[i][j][r];
+            if (splits > 1) ws[((long)bz * d.M + m) * d.N + n] = cplx<T>{xr, xi};
+            else store_out(m, n, xr, xi);
+          }
         }
   }
 }
@@ -327,8 +389,8 @@ __global__ __launch_bounds__(256 * KG) void gemm_kernel(GemmDesc d, const cplx<T
                                                         cplx<T>* __restrict__ C,
                                                         cplx<T>* __restrict__ ws, int splits,
                                                         int tiles_per_split) {
-  gemm_body<T, A_MCONT, B_NCONT, KG>(d, A, B, C, ws, splits, tiles_per_split, blockIdx.x, blockIdx.y,
-                                     blockIdx.z);
+  gemm_body<T, A_MCONT, B_NCONT, KG, false>(d, A, B, C, ws, splits, tiles_per_split, blockIdx.x, blockIdx.y,
+                                            blockIdx.z);
 }
 
 // Sum the split-K partial tiles in split order (deterministic) and apply
@@ -376,6 +438,10 @@ __global__ void splitk_reduce(GemmDesc d, const cplx<T>* __restrict__ ws, cplx<T
 // Launch decisions of one GEMM, a function of its own descriptor and
 // workspace only (so a GEMM computes the same whether launched alone or
 // grouped with others).
+// Output tiles of a grouped GEMM launch from which split GEMMs run fused (one
+// workgroup per output tile over all splits): four per CU.
+constexpr long GEMM_GROUP_FUSE_TILES = 1024;
+
 struct GemmPlan {
   bool empty = false;
   int tm = 0, tn = 0, batch = 1, splits = 1, kps = 0;
@@ -467,14 +533,14 @@ void gemm_launch(const GemmDesc& d, const cplx<T>* A, const cplx<T>* B, cplx<T>*
 }
 
 // ------------------------------------------------------------------ grouped
-template <typename T, bool A_MCONT, bool B_NCONT, int KG>
+template <typename T, bool A_MCONT, bool B_NCONT, int KG, bool FUSED>
 __global__ __launch_bounds__(256 * KG) void gemm_grouped_kernel(const GRec<GemmArgs<T>>* __restrict__ recs,
                                                                 int n, int j) {
   unsigned bx, by, bz;
   const GRec<GemmArgs<T>>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
   const GemmDesc d = r.a.d;
-  gemm_body<T, A_MCONT, B_NCONT, KG>(d, r.a.A, r.a.B + (long)j * r.a.b_jstride, r.a.C, r.a.ws, r.a.splits,
-                                     r.a.kps, bx, by, bz);
+  gemm_body<T, A_MCONT, B_NCONT, KG, FUSED>(d, r.a.A, r.a.B + (long)j * r.a.b_jstride, r.a.C, r.a.ws,
+                                            r.a.splits, r.a.kps, bx, by, bz);
 }
 
 template <typename T>
@@ -489,7 +555,23 @@ __global__ void splitk_reduce_grouped(const GRec<GemmArgs<T>>* __restrict__ recs
 
 template <typename T>
 void GemmGroup<T>::prepare(const std::vector<GemmJob<T>>& jobs, RecordArena& arena) {
-  std::vector<GRec<GemmArgs<T>>> var[8], red;
+  std::vector<GRec<GemmArgs<T>>> var[16], red;
+  // A GEMM's splits (and K-groups) are fixed by its own plan, so that it sums
+  // in the same order whether launched alone or grouped.  When the group's
+  // output tiles alone fill the GPU, each split GEMM runs one workgroup per
+  // output tile that walks all its splits and adds their partial tiles in
+  // split order (bit-identical to the split-K launch plus splitk_reduce),
+  // without the partial tiles' round trip through memory and the reduce.
+  long out_tiles = 0;
+  for (const auto& j : jobs) {
+    const GemmPlan p = gemm_plan(j.d, j.ws_elems);
+    if (!p.empty) out_tiles += (long)p.tm * p.tn * p.batch;
+  }
+  // QCA_GEMM_GROUP_FUSE_TILES overrides the threshold (0: always), so that the
+  // fused form can be tested against single launches on small groups
+  long fuse_tiles = GEMM_GROUP_FUSE_TILES;
+  if (const char* e = std::getenv("QCA_GEMM_GROUP_FUSE_TILES")) fuse_tiles = std::atol(e);
+  const bool fuse = out_tiles >= fuse_tiles;
   for (const auto& j : jobs) {
     const GemmPlan p = gemm_plan(j.d, j.ws_elems);
     if (p.empty) continue;
@@ -502,17 +584,18 @@ void GemmGroup<T>::prepare(const std::vector<GemmJob<T>>& jobs, RecordArena& are
     r.a.b_jstride = j.b_jstride;
     r.a.splits = p.splits;
     r.a.kps = p.kps;
+    r.a.fused = (fuse && p.splits > 1) ? 1 : 0;
     r.grid.x = p.tn;
     r.grid.y = p.tm;
-    r.grid.z = p.batch * p.splits;
-    var[(p.am ? 4 : 0) + (p.bn ? 2 : 0) + (p.kg2 ? 1 : 0)].push_back(r);
-    if (p.splits > 1) {
+    r.grid.z = r.a.fused ? p.batch : p.batch * p.splits;
+    var[(p.am ? 8 : 0) + (p.bn ? 4 : 0) + (p.kg2 ? 2 : 0) + r.a.fused].push_back(r);
+    if (p.splits > 1 && !r.a.fused) {
       r.grid = VGrid{};
       r.grid.x = p.reduce_blocks;
       red.push_back(r);
     }
   }
-  for (int v = 0; v < 8; ++v) {
+  for (int v = 0; v < 16; ++v) {
     launches[v].n = (int)var[v].size();
     launches[v].blocks = grec_layout(var[v]);
     launches[v].recs = var[v].empty() ? nullptr : arena.push(var[v]);
@@ -524,18 +607,26 @@ void GemmGroup<T>::prepare(const std::vector<GemmJob<T>>& jobs, RecordArena& are
 
 template <typename T>
 void GemmGroup<T>::launch(hipStream_t s, int j) const {
-#define QCA_GG_L(V, AM, BNC, KG)                                                               \
+#define QCA_GG_L(V, AM, BNC, KG, F)                                                            \
   if (launches[V].n)                                                                          \
-    hipLaunchKernelGGL((gemm_grouped_kernel<T, AM, BNC, KG>), dim3(launches[V].blocks),        \
+    hipLaunchKernelGGL((gemm_grouped_kernel<T, AM, BNC, KG, F>), dim3(launches[V].blocks),     \
                        dim3(256 * KG), 0, s, launches[V].recs, launches[V].n, j);
-  QCA_GG_L(0, false, false, 1)
-  QCA_GG_L(1, false, false, 2)
-  QCA_GG_L(2, false, true, 1)
-  QCA_GG_L(3, false, true, 2)
-  QCA_GG_L(4, true, false, 1)
-  QCA_GG_L(5, true, false, 2)
-  QCA_GG_L(6, true, true, 1)
-  QCA_GG_L(7, true, true, 2)
+  QCA_GG_L(0, false, false, 1, false)
+  QCA_GG_L(1, false, false, 1, true)
+  QCA_GG_L(2, false, false, 2, false)
+  QCA_GG_L(3, false, false, 2, true)
+  QCA_GG_L(4, false, true, 1, false)
+  QCA_GG_L(5, false, true, 1, true)
+  QCA_GG_L(6, false, true, 2, false)
+  QCA_GG_L(7, false, true, 2, true)
+  QCA_GG_L(8, true, false, 1, false)
+  QCA_GG_L(9, true, false, 1, true)
+  QCA_GG_L(10, true, false, 2, false)
+  QCA_GG_L(11, true, false, 2, true)
+  QCA_GG_L(12, true, true, 1, false)
+  QCA_GG_L(13, true, true, 1, true)
+  QCA_GG_L(14, true, true, 2, false)
+  QCA_GG_L(15, true, true, 2, true)
 #undef QCA_GG_L
   QCA_HIP(hipGetLastError());
   if (reduce.n) {

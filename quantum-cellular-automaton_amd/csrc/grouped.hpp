@@ -98,6 +98,7 @@ struct GemmArgs {
   cplx<T>* ws;
   long b_jstride;   // B advances by j * b_jstride elements in launch(s, j)
   int splits, kps;  // kps: K-tiles per split
+  int fused;        // one workgroup per output tile takes all splits (no partials, no reduce)
 };
 
 // One GEMM of one instance, with the split-K workspace it would use alone.
@@ -121,13 +122,13 @@ struct GLaunch {
   unsigned blocks = 0;
 };
 
-// A set of GEMMs (one per instance, any shapes) as at most eight grouped
+// A set of GEMMs (one per instance, any shapes) as at most sixteen grouped
 // launches (one per kernel variant) plus one grouped split-K reduction.
 // prepare() pushes the records; launch() may run any number of times once the
 // arena has been uploaded.
 template <typename T>
 struct GemmGroup {
-  GLaunch<GemmArgs<T>> launches[8];
+  GLaunch<GemmArgs<T>> launches[16];   // by (A m-contiguous, B n-contiguous, two K-groups, fused)
   GLaunch<GemmArgs<T>> reduce;
   void prepare(const std::vector<GemmJob<T>>& jobs, RecordArena& arena);
   void launch(hipStream_t s, int j = 0) const;

@@ -61,6 +61,30 @@ def test_group_is_bit_identical_to_engines_alone(gpu, prec):
         e.close()
 
 
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
+def test_group_fused_split_gemms_equal_engines_alone(gpu, prec, monkeypatch):
+    """Grouped GEMMs whose splits run fused (one workgroup per output tile
+    walking all K-splits, no partial tiles or reduce; forced here on a small
+    group, used by large groups) sum in the split-K launch's order: every
+    instance still ends bit-identical to the engine stepped alone."""
+    monkeypatch.setenv("QCA_GEMM_GROUP_FUSE_TILES", "0")
+    steps = 6
+    grouped, alone = draws_and_chain(prec), draws_and_chain(prec)
+    g = qn.Group(grouped)
+    g.step(steps)
+    for e in alone:
+        e.step(steps)
+    assert max(int(e.bond_dims().max()) for e in grouped) > 32
+    for b, a in zip(grouped, alone):
+        assert np.array_equal(b.bond_dims(), a.bond_dims())
+        assert np.array_equal(b.rho(), a.rho())
+        for x, y in zip(b.tensors(), a.tensors()):
+            assert np.array_equal(x, y)
+    g.close()
+    for e in grouped + alone:
+        e.close()
+
+
 def test_group_small_instances_match_oracle(gpu):
     """Grouped FP64 instances of the batched workload with at most 20 cells
     against the oracle (1e-10, identical bonds)."""
