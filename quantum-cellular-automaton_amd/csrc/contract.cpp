@@ -59,16 +59,17 @@ ContractPlan<T> heff_plan(int kind, const cplx<T>* x, cplx<T>* y, const cplx<T>*
   const int dq = kind == 0 ? 4 : 2;
   QCA_CHECK(W->group_rows == Dout && W->nout == dq * Dout && W->nin == dq * Din,
             "sparse MPO does not match the block");
-  // GEMM1: T[q][(w,x)][c] = sum_a L[a][(w,x)] x[q][a][c]
-  GemmDesc g = gemm_desc(Din * chil, chir, chil);
-  g.NB2 = dq;
-  g.a_m = 1; g.a_k2 = (long)Din * chil;
-  g.b_k2 = chir; g.b_n = 1; g.b_b2 = lr;
-  g.c_m = chir; g.c_n = 1; g.c_b2 = (long)Din * lr;
+  // GEMM1, one batch per (q, w): T[q][w][c][x] = sum_a x[q][a][c] L[a][w][x],
+  // so that the second GEMM reads T's slices with x contiguous (its M index)
+  GemmDesc g = gemm_desc(chir, chil, chil);
+  g.NB1 = dq; g.NB2 = Din;
+  g.a_m = 1; g.a_k2 = chir; g.a_b1 = lr;
+  g.b_k2 = (long)Din * chil; g.b_n = 1; g.b_b2 = chil;
+  g.c_m = chil; g.c_n = 1; g.c_b1 = (long)Din * lr; g.c_b2 = lr;
   g.skip = skip;
-  p.g1 = GemmJob<T>{g, L, x, ws.T1, ws.splitk, ws.splitk_elems};
+  p.g1 = GemmJob<T>{g, x, L, ws.T1, ws.splitk, ws.splitk_elems};
   // GEMM3 with the MPO mixing gathered into its K loop:
-  //   y[q'][x][y] = sum_{(q,w,v) in W[.][q'][.][.]} W[q][q'][w][v] sum_c T[q][w][x][c] R[c][v][y]
+  //   y[q'][x][y] = sum_{(q,w,v) in W[.][q'][.][.]} W[q][q'][w][v] sum_c T[q][w][c][x] R[c][v][y]
   // one K segment per nonzero of the column block q' (seg_col = q*Din + w,
   // seg_outv = v), each a sum over c.
   p.mix = false;
@@ -81,7 +82,7 @@ ContractPlan<T> heff_plan(int kind, const cplx<T>* x, cplx<T>* y, const cplx<T>*
   h.seg_col = W->col;
   h.seg_outv = W->outv;
   h.seg_val = W->val;
-  h.a_m = chir; h.a_k1 = lr; h.a_k2 = 1;
+  h.a_m = 1; h.a_k1 = lr; h.a_k2 = chil;
   h.b_k1 = chir; h.b_k2 = (long)Dout * chir; h.b_n = 1;
   h.c_m = chir; h.c_n = 1; h.c_b2 = lr;
   h.skip = skip;

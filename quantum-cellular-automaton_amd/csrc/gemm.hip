@@ -174,6 +174,11 @@ __device__ __forceinline__ void gemm_body(const GemmDesc& d, const cplx<T>* __re
   // registers one iteration ahead of their LDS store
   cplx<T> pa[LOADS], pb[LOADS];
   cplx<T> wgt{T(1), T(0)};
+  // the gather form's segment record, looked up again only when a tile starts
+  // a new segment (one lookup per K2/BK tiles instead of one per tile)
+  int cached_seg = -1;
+  long cached_soa = 0, cached_sob = 0;
+  cplx<T> cached_wgt{T(1), T(0)};
   auto load_tile = [&](int t) {
     if (t < 0) {  // uniform per K-group: no memory access
 #pragma unroll
@@ -184,10 +189,16 @@ __device__ __forceinline__ void gemm_body(const GemmDesc& d, const cplx<T>* __re
     const int k0 = (t - seg * tps) * BK;
     long soa, sob;
     if (gather) {
-      const int e = e0 + seg;
-      soa = (long)d.seg_col[e] * d.a_k1;
-      sob = (long)d.seg_outv[e] * d.b_k1;
-      wgt = reinterpret_cast<const cplx<T>*>(d.seg_val)[e];
+      if (seg != cached_seg) {
+        const int e = e0 + seg;
+        cached_seg = seg;
+        cached_soa = (long)d.seg_col[e] * d.a_k1;
+        cached_sob = (long)d.seg_outv[e] * d.b_k1;
+        cached_wgt = reinterpret_cast<const cplx<T>*>(d.seg_val)[e];
+      }
+      soa = cached_soa;
+      sob = cached_sob;
+      wgt = cached_wgt;
     } else {
       soa = (long)seg * d.a_k1;
       sob = (long)seg * d.b_k1;
@@ -539,7 +550,8 @@ __global__ __launch_bounds__(256 * KG) void gemm_grouped_kernel(const GRec<GemmA
   unsigned bx, by, bz;
   const GRec<GemmArgs<T>>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
   const GemmDesc d = r.a.d;
-  gemm_body<T, A_MCONT, B_NCONT, KG, FUSED>(d, r.a.A, r.a.B + (long)j * r.a.b_jstride, r.a.C, r.a.ws,
+  gemm_body<T, A_MCONT, B_NCONT, KG, FUSED>(d, r.a.A + (long)j * r.a.a_jstride, r.a.B + (long)j * r.a.b_jstride,
+                                            r.a.C, r.a.ws,
                                             r.a.splits, r.a.kps, bx, by, bz);
 }
 
@@ -581,6 +593,7 @@ void GemmGroup<T>::prepare(const std::vector<GemmJob<T>>& jobs, RecordArena& are
     r.a.B = j.B;
     r.a.C = j.C;
     r.a.ws = j.ws;
+    r.a.a_jstride = j.a_jstride;
     r.a.b_jstride = j.b_jstride;
     r.a.splits = p.splits;
     r.a.kps = p.kps;

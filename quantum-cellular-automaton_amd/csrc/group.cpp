@@ -436,7 +436,7 @@ void Group::evolve(std::vector<int> inst, std::vector<int> kind, std::vector<int
       z.w = e->y64_;
       z.v32 = nullptr;
       p = heff_plan<T>(k, e->lw_.V, e->y64_, L, R, W, Din, Dout, chil[i], chir[i], e->ws64_, skip);
-      p.g1.b_jstride = n[i];   // the matvec reads basis vector V_j
+      p.g1.a_jstride = n[i];   // the matvec reads basis vector V_j (the first GEMM's A)
     }
     g1.push_back(p.g1);
     if (p.mix) {

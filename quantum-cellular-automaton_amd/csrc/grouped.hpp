@@ -96,14 +96,15 @@ struct GemmArgs {
   const cplx<T>* B;
   cplx<T>* C;
   cplx<T>* ws;
-  long b_jstride;   // B advances by j * b_jstride elements in launch(s, j)
+  long a_jstride, b_jstride;   // A, B advance by j * a_jstride, j * b_jstride elements in launch(s, j)
   int splits, kps;  // kps: K-tiles per split
   int fused;        // one workgroup per output tile takes all splits (no partials, no reduce)
 };
 
 // One GEMM of one instance, with the split-K workspace it would use alone.
-// b_jstride: the B operand of iteration j of a repeated launch is
-// B + j * b_jstride (a Krylov basis vector), 0 for a fixed operand.
+// a_jstride, b_jstride: the A (B) operand of iteration j of a repeated launch
+// is A + j * a_jstride (B + j * b_jstride): a Krylov basis vector; 0 for a
+// fixed operand.
 template <typename T>
 struct GemmJob {
   GemmDesc d;
@@ -113,6 +114,7 @@ struct GemmJob {
   cplx<T>* ws;
   long ws_elems;
   long b_jstride = 0;
+  long a_jstride = 0;
 };
 
 template <typename R>
