@@ -40,6 +40,7 @@ import ctypes
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -122,9 +123,10 @@ def batch_bench(torch, a):
     """Aggregate 2TDVP site-updates/s of the first `a.batch` instances of the
     batched workload (mixed precision), advanced one step at a time by
     `a.batch_threads` native workers, one stream per instance; the same engines
-    stepped one after another on one host thread give the sequential rate, and
-    plain FP64 engines started from the same warmed states, stepped one after
-    another, the rate of the plain path."""
+    stepped one after another on one host thread give the sequential rate,
+    stepped as one group per size lane the grouped rate, and plain FP64
+    engines started from the same warmed states, stepped one after another,
+    the rate of the plain path."""
     import qca_native as qn
     import workloads
     insts = workloads.batch_instances(256, seed=4)[:a.batch]
@@ -138,10 +140,36 @@ def batch_bench(torch, a):
     log(f"batched workload: warm after {a.batch_warmup} steps, max bond per instance "
         f"median {int(np.median(bonds))} max {max(bonds)}")
     rates = {}
+    # grouped: one group per size lane (serve.size_lane), the lanes' groups
+    # stepped side by side on their own host threads
+    import serve
+    lane_members: dict = {}
+    for e, i in zip(engines, insts):
+        lane_members.setdefault(serve.size_lane(i), []).append(e)
+
+    def grouped_run():
+        groups = [qn.Group(es) for es in lane_members.values()]
+        errors = []
+
+        def one(g):
+            try:
+                g.step(a.batch_steps)
+            except BaseException as exc:  # re-raised below, after every lane has stopped
+                errors.append(exc)
+        threads = [threading.Thread(target=one, args=(g,)) for g in groups]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        for g in groups:
+            g.close()
+        if errors:
+            raise errors[0]
     # one instance at a time favours latency (one-launch Krylov up to 1024
     # elements); many in flight favour throughput (up to 4096)
     for label, limit, run in (("sequential", 1024, lambda: [e.step(a.batch_steps) for e in engines]),
-                              ("batched", 4096, lambda: qn.batch_step(engines, a.batch_steps, a.batch_threads))):
+                              ("batched", 4096, lambda: qn.batch_step(engines, a.batch_steps, a.batch_threads)),
+                              ("grouped", 4096, grouped_run)):
         for e in engines:
             e.set_fused_krylov_limit(limit)
             e.reset_stats()
@@ -181,7 +209,10 @@ def batch_bench(torch, a):
             "site_updates_per_s": rates["batched"], "sequential_site_updates_per_s": rates["sequential"],
             "speedup_over_sequential": rates["batched"] / rates["sequential"],
             "fp64_sequential_site_updates_per_s": rates["fp64_sequential"],
-            "speedup_over_fp64_sequential": rates["batched"] / rates["fp64_sequential"]}
+            "speedup_over_fp64_sequential": rates["batched"] / rates["fp64_sequential"],
+            "grouped": {"lanes": len(lane_members), "site_updates_per_s": rates["grouped"],
+                        "speedup_over_fp64_sequential": rates["grouped"] / rates["fp64_sequential"],
+                        "method": "one group per size lane (qca_group_step), lanes on their own threads"}}
 
 
 def batch128_bench(torch, a, tensors, fp64_rate):
