@@ -10,6 +10,8 @@
 //   jac_inner  one workgroup per pair sums its partial Grams and runs one
 //              cyclic sweep of two-sided Jacobi rotations on G_p in LDS
 //              (G <- R^H G R), accumulating J_p = product of the rotations;
+//              two rounds of a sweep sweep all pairs of the 2bs columns,
+//              the others only those across the two blocks;
 //   jac_apply  X_p <- X_p J_p, each workgroup one chunk of rows, on the
 //              FP64 matrix cores.
 // Stream order separates the rounds.  The Gram is rebuilt from the columns in
@@ -191,13 +193,32 @@ __device__ __forceinline__ int rr_fast(int t, int r, int n) {
   x -= (x >= n - 1) ? (n - 1) : 0;
   return x + 1;
 }
+// Pair t of inner round r.  Full ordering: the round-robin tournament over all
+// L = 2H columns (L - 1 rounds, every pair once).  Cross ordering: column t of
+// the first block meets column H + (t + r) mod H of the second (H rounds, only
+// the pairs across the two blocks).
+template <bool CROSS>
+__device__ __forceinline__ int pair_first(int t, int r, int L) {
+  return CROSS ? t : rr_fast(t, r, L);
+}
+template <bool CROSS>
+__device__ __forceinline__ int pair_second(int t, int r, int L) {
+  if (!CROSS) return rr_fast(L - 1 - t, r, L);
+  const int H = L >> 1;
+  int x = t + r;
+  x -= (x >= H) ? H : 0;
+  return H + x;
+}
 // One cyclic sweep of the rotations on G (L x L, column-major, in LDS),
-// accumulating J <- J R; rot is a 2 x (L/2) LDS scratch.  Needs blockDim.x ==
-// JAC_NT and G, J visible to the whole workgroup on entry (it ends with a
+// accumulating J <- J R; rot is a 2 x (L/2) LDS scratch.  CROSS restricts the
+// sweep to the pairs across the two halves (see pair_first).  Needs blockDim.x
+// == JAC_NT and G, J visible to the whole workgroup on entry (it ends with a
 // barrier).  Returns the rotations this thread computed.
+template <bool CROSS>
 __device__ int inner_sweep(c128* G, c128* J, JRot (*rot)[JAC_LMAX / 2], int L, int H, double tol2,
                            double negl) {
   const int tid = threadIdx.x;
+  const int NR = CROSS ? H : L - 1;
   int mine = 0;
   // fixed roles: lanes < H of wave 0 compute the rotations; threads 64.. own
   // one J item (row r, pair Q) each; threads < H*H own one 2x2 block of G
@@ -206,11 +227,11 @@ __device__ int inner_sweep(c128* G, c128* J, JRot (*rot)[JAC_LMAX / 2], int L, i
   const int jr = has_j ? jit % L : 0, jq = has_j ? jit / L : 0;
   const bool has_g = tid < H * H;
   const int gP = tid % H, gQ = tid / H;
-  for (int round = 0; round < L; ++round) {
+  for (int round = 0; round <= NR; ++round) {
     const int cur = round & 1, prev = cur ^ 1;
     if (tid < 64) {
-      if (round < L - 1 && tid < H) {
-        const int i = rr_fast(tid, round, L), j = rr_fast(L - 1 - tid, round, L);
+      if (round < NR && tid < H) {
+        const int i = pair_first<CROSS>(tid, round, L), j = pair_second<CROSS>(tid, round, L);
         const double a = G[i * L + i].re, b = G[j * L + j].re;
         const c128 gij = G[j * L + i];
         const double ag2 = gij.re * gij.re + gij.im * gij.im;
@@ -232,7 +253,7 @@ __device__ int inner_sweep(c128* G, c128* J, JRot (*rot)[JAC_LMAX / 2], int L, i
         rot[cur][tid] = q;
       }
     } else if (round > 0 && has_j) {
-      const int qi = rr_fast(jq, round - 1, L), qj = rr_fast(L - 1 - jq, round - 1, L);
+      const int qi = pair_first<CROSS>(jq, round - 1, L), qj = pair_second<CROSS>(jq, round - 1, L);
       const c128 x = J[qi * L + jr], y = J[qj * L + jr];
       const JRot Q = rot[prev][jq];
       if (Q.act) {
@@ -241,10 +262,10 @@ __device__ int inner_sweep(c128* G, c128* J, JRot (*rot)[JAC_LMAX / 2], int L, i
       }
     }
     __syncthreads();
-    if (round == L - 1) break;
+    if (round == NR) break;
     if (has_g) {
-      const int pi = rr_fast(gP, round, L), pj = rr_fast(L - 1 - gP, round, L);
-      const int qi = rr_fast(gQ, round, L), qj = rr_fast(L - 1 - gQ, round, L);
+      const int pi = pair_first<CROSS>(gP, round, L), pj = pair_second<CROSS>(gP, round, L);
+      const int qi = pair_first<CROSS>(gQ, round, L), qj = pair_second<CROSS>(gQ, round, L);
       const c128 t00 = G[qi * L + pi], t10 = G[qi * L + pj], t01 = G[qj * L + pi], t11 = G[qj * L + pj];
       const JRot P = rot[cur][gP], Q = rot[cur][gQ];
       if (P.act || Q.act) {
@@ -263,7 +284,15 @@ __device__ int inner_sweep(c128* G, c128* J, JRot (*rot)[JAC_LMAX / 2], int L, i
   return mine;
 }
 
-__device__ __forceinline__ void jac_inner_body(const JacobiGeom& g, int sweep, const c128* __restrict__ Gpart,
+// Outer rounds 0 and (nb - 1) / 2 of a sweep run the full inner ordering, which
+// also visits the pairs inside each block (every block takes part in every
+// outer round); the other rounds visit only the pairs across their two blocks.
+// A sweep thus meets every column pair at least once, in 2 * 31 + 13 * 16 inner
+// rounds for 16 blocks of 16 instead of 15 * 31.  Inside pairs once per sweep
+// (round 0 only) cost about one sweep more; twice keeps the sweep count of the
+// full ordering within a fraction of a sweep.
+__device__ __forceinline__ void jac_inner_body(const JacobiGeom& g, int round, int sweep,
+                                               const c128* __restrict__ Gpart,
                                                c128* __restrict__ Jout, int* __restrict__ rotated,
                                                double tol, JacobiCtrl* __restrict__ ctrl, int p) {
   if (ctrl->done) return;
@@ -286,7 +315,9 @@ __device__ __forceinline__ void jac_inner_body(const JacobiGeom& g, int sweep, c
   }
   if (tid == 0) total = 0;
   __syncthreads();
-  const int mine = inner_sweep(G, J, rot, L, H, tol * tol, negl);
+  const bool cross = round != 0 && round != (g.nb - 1) / 2;
+  const int mine = cross ? inner_sweep<true>(G, J, rot, L, H, tol * tol, negl)
+                         : inner_sweep<false>(G, J, rot, L, H, tol * tol, negl);
   if (mine) atomicAdd(&total, mine);
   __syncthreads();
   c128* jo = Jout + (long)p * L * L;
@@ -296,12 +327,13 @@ __device__ __forceinline__ void jac_inner_body(const JacobiGeom& g, int sweep, c
     if (total) atomicAdd(&ctrl->rot[sweep], total);
   }
 }
-__global__ __launch_bounds__(JAC_NT) void jac_inner(JacobiGeom g, int sweep,
+// Two workgroups per CU (64 VGPRs): grouped launches hold a pair per workgroup.
+__global__ __launch_bounds__(JAC_NT) __attribute__((amdgpu_waves_per_eu(8))) void jac_inner(JacobiGeom g, int round, int sweep,
                                                     const c128* __restrict__ Gpart,
                                                     c128* __restrict__ Jout,
                                                     int* __restrict__ rotated, double tol,
                                                     JacobiCtrl* __restrict__ ctrl) {
-  jac_inner_body(g, sweep, Gpart, Jout, rotated, tol, ctrl, blockIdx.x);
+  jac_inner_body(g, round, sweep, Gpart, Jout, rotated, tol, ctrl, blockIdx.x);
 }
 
 // X_p <- X_p J_p on row chunk blockIdx.y of pair blockIdx.x; chunks from
@@ -497,7 +529,7 @@ __device__ __forceinline__ void jac_small_body(c128* __restrict__ X, int rows, i
     }
     if (tid == 0) total = 0;
     __syncthreads();
-    const int mine = inner_sweep(G, J, rot, L, H, tol * tol, negl);
+    const int mine = inner_sweep<false>(G, J, rot, L, H, tol * tol, negl);
     if (mine) atomicAdd(&total, mine);
     __syncthreads();
     if (total == 0) {
@@ -1007,13 +1039,13 @@ __global__ __launch_bounds__(JAC_GRAM_NT) void jac_gram_grouped(const GRec<JacAr
   if (round >= g.nb - 1) return;
   jac_gram_body(r.a.Y, g, round, r.a.Gpart, r.a.ctrl, bx, by);
 }
-__global__ __launch_bounds__(JAC_NT) void jac_inner_grouped(const GRec<JacArgs>* __restrict__ recs, int n,
+__global__ __launch_bounds__(JAC_NT) __attribute__((amdgpu_waves_per_eu(8))) void jac_inner_grouped(const GRec<JacArgs>* __restrict__ recs, int n,
                                                             int round, int sweep) {
   unsigned bx, by, bz;
   const GRec<JacArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
   const JacobiGeom g = r.a.g;
   if (round >= g.nb - 1) return;
-  jac_inner_body(g, sweep, r.a.Gpart, r.a.Jbuf, r.a.rotated, r.a.tol, r.a.ctrl, bx);
+  jac_inner_body(g, round, sweep, r.a.Gpart, r.a.Jbuf, r.a.rotated, r.a.tol, r.a.ctrl, bx);
 }
 __global__ __launch_bounds__(JAC_APPLY_NT) void jac_apply_grouped(const GRec<JacArgs>* __restrict__ recs, int n,
                                                             int round) {
@@ -1091,7 +1123,7 @@ JacobiCtrl* gram_jacobi(c128* Y, int rows, int cols, c128* Z, int zrows, hipStre
       for (int r = 0; r < l.nb - 1; ++r) {
         hipLaunchKernelGGL(jac_gram, dim3(np, l.kspl), dim3(JAC_GRAM_NT), 0, s, (const c128*)Y, g, r,
                            Gpart, (const JacobiCtrl*)ctrl);
-        hipLaunchKernelGGL(jac_inner, dim3(np), dim3(JAC_NT), 0, s, g, sweep, (const c128*)Gpart,
+        hipLaunchKernelGGL(jac_inner, dim3(np), dim3(JAC_NT), 0, s, g, r, sweep, (const c128*)Gpart,
                            Jbuf, rotated, tol, ctrl);
         hipLaunchKernelGGL(jac_apply, dim3(np, l.kspl + gz.kspl), dim3(JAC_APPLY_NT), 0, s, Y, g, Z, gz, r,
                            (const c128*)Jbuf, (const int*)rotated, (const JacobiCtrl*)ctrl);
