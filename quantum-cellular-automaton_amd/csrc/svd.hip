@@ -51,6 +51,15 @@ constexpr int JAC_MAX_SWEEPS = 60;
 // their singular values are at the accuracy floor of the method, far below any
 // truncation cutoff the engine is used with (1e-10).
 constexpr double JAC_NOISE2 = 256.0 * 1.1102230246251565e-16 * 1.1102230246251565e-16;
+// A pair is rotated when |g_ij| > tol sqrt(g_ii g_jj); a sweep counts as
+// converged when no rotation had |g_ij| > conv sqrt(g_ii g_jj), conv =
+// max(tol, 4 eps sqrt(rows)).  At the working-precision tolerance eps sqrt(rows)
+// the inner products of freshly rotated columns carry rounding noise of about
+// that size, so without the margin a pair can sit just above the threshold
+// sweep after sweep; its last rotations are still applied.
+__device__ __forceinline__ double jac_conv_tol(double tol, int rows) {
+  return fmax(tol, 4.0 * 1.1102230246251565e-16 * sqrt((double)rows));
+}
 
 __device__ __forceinline__ double wsum(double v) {
 #pragma unroll
@@ -172,9 +181,9 @@ __global__ __launch_bounds__(JAC_GRAM_NT) void jac_gram(const c128* __restrict__
 //   c = sqrt(w / (2h)),  s e = sign(d) g sqrt(2 / (h w)),
 // the Rutishauser rotation t = sign(d)/(|zeta| + sqrt(1 + zeta^2)), zeta = d/(2|g|),
 // c = 1/sqrt(1 + t^2), s = t c, rewritten so that c^2 + |s e|^2 = 1 exactly in
-// real arithmetic, every operation is a correctly rounded add, multiply,
-// divide or square root on non-negative operands, and the critical path is
-// sqrt -> divide -> sqrt.  G <- R^H G R and J <- J R.
+// real arithmetic on non-negative operands; c and |s| are evaluated as w y and
+// 2 y with y = 1/sqrt(2hw), so the critical path is sqrt -> rsqrt.
+// G <- R^H G R and J <- J R.
 // Per round: wave 0 computes the rotations of round r while the other waves
 // apply round r-1 to J; after a barrier all threads apply round r to G.
 // The pair indices are recomputed in registers, the rotation of a pair is one
@@ -216,7 +225,7 @@ __device__ __forceinline__ int pair_second(int t, int r, int L) {
 // barrier).  Returns the rotations this thread computed.
 template <bool CROSS>
 __device__ int inner_sweep(c128* G, c128* J, JRot (*rot)[JAC_LMAX / 2], int L, int H, double tol2,
-                           double negl) {
+                           double conv2, double negl) {
   const int tid = threadIdx.x;
   const int NR = CROSS ? H : L - 1;
   int mine = 0;
@@ -244,11 +253,26 @@ __device__ int inner_sweep(c128* G, c128* J, JRot (*rot)[JAC_LMAX / 2], int L, i
           const double d = b - a;
           const double h = sqrt(d * d + 4.0 * ag2);
           const double w = h + fabs(d);
-          q.c = sqrt(w / (2.0 * h));
-          const double s = (d >= 0 ? 1.0 : -1.0) * sqrt(2.0 / (h * w));
+          // c = sqrt(w / 2h) = w y and |s| = sqrt(2 / hw) = 2 y with y =
+          // 1/sqrt(2hw): one reciprocal square root in place of two divides
+          // and two square roots on the serial path of the round.  Hardware
+          // estimate, two Newton steps, and a last one whose residual
+          // 1 - t y^2 includes the exact error of t y, so that y carries no
+          // bias (c and s share y: a biased y drifts the column norms)
+          const double t = 2.0 * h * w;
+          double y = __builtin_amdgcn_rsq(t);
+          y = y * fma(-0.5 * t * y, y, 1.5);
+          y = y * fma(-0.5 * t * y, y, 1.5);
+          const double ty = t * y;
+          const double e = fma(-ty, y, 1.0) - fma(t, y, -ty) * y;
+          y = fma(0.5 * y, e, y);
+          q.c = w * y;
+          const double s = (d >= 0 ? 2.0 : -2.0) * y;
           q.sr = s * gij.re;
           q.si = s * gij.im;
-          ++mine;
+          // low half: rotations applied; high half: rotations above the
+          // convergence threshold, which keep the sweeps going
+          mine += ag2 > conv2 * a * b ? 0x10001 : 1;
         }
         rot[cur][tid] = q;
       }
@@ -300,9 +324,10 @@ __device__ __forceinline__ void jac_inner_body(const JacobiGeom& g, int round, i
   __shared__ c128 G[JAC_LMAX * JAC_LMAX];
   __shared__ c128 J[JAC_LMAX * JAC_LMAX];
   __shared__ JRot rot[2][HM];
-  __shared__ int total;
+  __shared__ int total, total_sig;
   const int L = 2 * g.bs, H = g.bs, tid = threadIdx.x;
   const double negl = JAC_NOISE2 * ctrl->fro;
+  const double conv = jac_conv_tol(tol, g.rows);
   for (int e = tid; e < L * L; e += JAC_NT) {
     double re = 0, im = 0;
     for (int kc = 0; kc < g.kspl; ++kc) {
@@ -313,18 +338,22 @@ __device__ __forceinline__ void jac_inner_body(const JacobiGeom& g, int round, i
     G[e] = c128{re, im};
     J[e] = c128{(e % L) == (e / L) ? 1.0 : 0.0, 0.0};
   }
-  if (tid == 0) total = 0;
+  if (tid == 0) {
+    total = 0;
+    total_sig = 0;
+  }
   __syncthreads();
   const bool cross = round != 0 && round != (g.nb - 1) / 2;
-  const int mine = cross ? inner_sweep<true>(G, J, rot, L, H, tol * tol, negl)
-                         : inner_sweep<false>(G, J, rot, L, H, tol * tol, negl);
-  if (mine) atomicAdd(&total, mine);
+  const int mine = cross ? inner_sweep<true>(G, J, rot, L, H, tol * tol, conv * conv, negl)
+                         : inner_sweep<false>(G, J, rot, L, H, tol * tol, conv * conv, negl);
+  if (mine & 0xffff) atomicAdd(&total, mine & 0xffff);
+  if (mine >> 16) atomicAdd(&total_sig, mine >> 16);
   __syncthreads();
   c128* jo = Jout + (long)p * L * L;
   for (int e = tid; e < L * L; e += JAC_NT) jo[e] = J[e];
   if (tid == 0) {
     rotated[p] = total;
-    if (total) atomicAdd(&ctrl->rot[sweep], total);
+    if (total_sig) atomicAdd(&ctrl->rot[sweep], total_sig);
   }
 }
 // Two workgroups per CU (64 VGPRs): grouped launches hold a pair per workgroup.
@@ -494,9 +523,10 @@ __device__ __forceinline__ void jac_small_body(c128* __restrict__ X, int rows, i
   __shared__ c128 J[JAC_LMAX * JAC_LMAX];
   __shared__ JRot rot[2][HM];
   __shared__ double red[JAC_NT / 64];
-  __shared__ int total;
+  __shared__ int total, total_sig;
   const int tid = threadIdx.x;
   const int H = (cols + 1) / 2, L = 2 * H;
+  const double conv = jac_conv_tol(tol, rows);
   double part = 0;
   for (long e = tid; e < (long)rows * cols; e += JAC_NT) part += X[e].re * X[e].re + X[e].im * X[e].im;
   part = wsum(part);
@@ -527,10 +557,14 @@ __device__ __forceinline__ void jac_small_body(c128* __restrict__ X, int rows, i
       G[tid] = c128{gre, gim};
       J[tid] = c128{gi == gj ? 1.0 : 0.0, 0.0};
     }
-    if (tid == 0) total = 0;
+    if (tid == 0) {
+      total = 0;
+      total_sig = 0;
+    }
     __syncthreads();
-    const int mine = inner_sweep<false>(G, J, rot, L, H, tol * tol, negl);
-    if (mine) atomicAdd(&total, mine);
+    const int mine = inner_sweep<false>(G, J, rot, L, H, tol * tol, conv * conv, negl);
+    if (mine & 0xffff) atomicAdd(&total, mine & 0xffff);
+    if (mine >> 16) atomicAdd(&total_sig, mine >> 16);
     __syncthreads();
     if (total == 0) {
       converged = 1;
@@ -553,6 +587,10 @@ __device__ __forceinline__ void jac_small_body(c128* __restrict__ X, int rows, i
           X[(long)c * rows + r0 + r] = c128{re, im};
         }
       __syncthreads();
+    }
+    if (total_sig == 0) {
+      converged = 1;
+      break;
     }
   }
   __syncthreads();
