@@ -8,6 +8,7 @@
 //  * measure   -- <n_i> and the single-site density matrix from the transfer
 //  * exact     -- closed-form 2^N rule-Hamiltonian application
 #include "grouped.hpp"
+#include "wave.hpp"
 
 #include <cstring>
 
@@ -24,11 +25,7 @@ __device__ __forceinline__ cplx<T> cmulc(cplx<T> a, cplx<T> b) {  // conj(a) * b
   return cplx<T>{a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
+__device__ __forceinline__ double wave_sum(double v) { return wave_sum64(v); }
 
 // Block-wide sum of a double; blockDim.x must be a multiple of 64, <= 1024.
 __device__ double block_sum(double v, double* sh) {

@@ -23,6 +23,7 @@
 // they fit (environments, sparse MPO, both intermediates, then leading basis
 // vectors) and the kernel reaches each one through a generic pointer.
 #include "grouped.hpp"
+#include "wave.hpp"
 
 namespace qca {
 namespace {
@@ -33,11 +34,7 @@ constexpr int FK_EPT = FUSED_KRYLOV_MAX_N / FK_THREADS;  // residual elements pe
 // Dynamic LDS of one launch (the static part, FkShared, adds ~3 KB; 160 KB per CU).
 constexpr long FK_LDS_BYTES = 152 * 1024;
 
-__device__ __forceinline__ double fk_wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
+__device__ __forceinline__ double fk_wave_sum(double v) { return wave_sum64(v); }
 
 // Element i of p in the compute type TC.
 template <typename TC, typename TS>

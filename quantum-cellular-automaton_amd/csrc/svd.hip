@@ -32,6 +32,7 @@
 // beta = -sign(Re alpha)*|x|, so R has a real diagonal and Q matches LAPACK's
 // complete-QR complement columns.
 #include "grouped.hpp"
+#include "wave.hpp"
 
 #include <algorithm>
 
@@ -61,11 +62,7 @@ __device__ __forceinline__ double jac_conv_tol(double tol, int rows) {
   return fmax(tol, 4.0 * 1.1102230246251565e-16 * sqrt((double)rows));
 }
 
-__device__ __forceinline__ double wsum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
+__device__ __forceinline__ double wsum(double v) { return wave_sum64(v); }
 
 // Position t of round r in the round-robin tournament over n (even) players.
 __device__ __forceinline__ int rr_player(int t, int r, int n) {
