@@ -544,23 +544,33 @@ void gemm_launch(const GemmDesc& d, const cplx<T>* A, const cplx<T>* B, cplx<T>*
 }
 
 // ------------------------------------------------------------------ grouped
+// The descriptor's pointers in the global address space (see gbl).
+__device__ __forceinline__ GemmDesc global_desc(GemmDesc d) {
+  d.skip = gbl(d.skip);
+  d.seg_rowptr = gbl(d.seg_rowptr);
+  d.seg_col = gbl(d.seg_col);
+  d.seg_outv = gbl(d.seg_outv);
+  d.seg_val = gbl(d.seg_val);
+  return d;
+}
+
 template <typename T, bool A_MCONT, bool B_NCONT, int KG, bool FUSED>
 __global__ __launch_bounds__(256 * KG) void gemm_grouped_kernel(const GRec<GemmArgs<T>>* __restrict__ recs,
                                                                 int n, int j) {
   unsigned bx, by, bz;
   const GRec<GemmArgs<T>>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
-  const GemmDesc d = r.a.d;
-  gemm_body<T, A_MCONT, B_NCONT, KG, FUSED>(d, r.a.A + (long)j * r.a.a_jstride, r.a.B + (long)j * r.a.b_jstride,
-                                            r.a.C, r.a.ws,
-                                            r.a.splits, r.a.kps, bx, by, bz);
+  const GemmDesc d = global_desc(r.a.d);
+  gemm_body<T, A_MCONT, B_NCONT, KG, FUSED>(d, gbl(r.a.A + (long)j * r.a.a_jstride),
+                                            gbl(r.a.B + (long)j * r.a.b_jstride), gbl(r.a.C),
+                                            gbl(r.a.ws), r.a.splits, r.a.kps, bx, by, bz);
 }
 
 template <typename T>
 __global__ void splitk_reduce_grouped(const GRec<GemmArgs<T>>* __restrict__ recs, int n) {
   unsigned bx, by, bz;
   const GRec<GemmArgs<T>>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
-  const GemmDesc d = r.a.d;
-  splitk_reduce_body<T>(d, r.a.ws, r.a.C, r.a.splits, d.NB1 * d.NB2, bx, r.grid.x);
+  const GemmDesc d = global_desc(r.a.d);
+  splitk_reduce_body<T>(d, gbl(r.a.ws), gbl(r.a.C), r.a.splits, d.NB1 * d.NB2, bx, r.grid.x);
 }
 
 }  // namespace

@@ -29,6 +29,23 @@ struct GRec {
   unsigned pad;
 };
 
+// Buffers named in records are device (global) memory, but a pointer read
+// from memory is generic to the compiler: it emits flat accesses, which count
+// against both the vector-memory and the LDS wait counters and are ordered
+// against LDS traffic.  Rebuilding the pointer in the global address space
+// lets the grouped kernels use the same global accesses as the
+// single-instance ones.  p must be uniform over the workgroup (a record's
+// pointers are); the readfirstlane keeps the compiler from folding the
+// round trip away.
+template <typename T>
+__device__ __forceinline__ T* gbl(T* p) {
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned long long u =
+      ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(v >> 32)) << 32) |
+      (unsigned)__builtin_amdgcn_readfirstlane((unsigned)v);
+  return (T*)(__attribute__((address_space(1))) T*)u;
+}
+
 // Finds the record holding flat block `flat` (records sorted by begin) and
 // the block's virtual index within it.
 template <typename A>

@@ -111,8 +111,8 @@ __global__ __launch_bounds__(WMIX_XC * WMIX_WAVES) void wmix_grouped(const GRec<
   unsigned bx, by, bz;
   const GRec<WmixArgs<T>>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
   const WmixArgs<T> a = r.a;
-  wmix_body<T>(a.Tin, a.in_stride, a.nxc, a.nin, a.U, a.nq, a.Dout, a.rowptr, a.col, a.val, a.skip, bx,
-               smem_raw);
+  wmix_body<T>(gbl(a.Tin), a.in_stride, a.nxc, a.nin, gbl(a.U), a.nq, a.Dout, gbl(a.rowptr), gbl(a.col),
+               gbl(a.val), gbl(a.skip), bx, smem_raw);
 }
 
 // ------------------------------------------------------------------ conversions
@@ -132,7 +132,7 @@ __global__ void convert_kernel(const cplx<S>* __restrict__ in, cplx<D>* __restri
 __global__ void convert_grouped(const GRec<ConvArgs>* __restrict__ recs, int n) {
   unsigned bx, by, bz;
   const GRec<ConvArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
-  convert_body<double, float>(r.a.in, r.a.out, r.a.n, r.a.skip, bx, r.grid.x);
+  convert_body<double, float>(gbl(r.a.in), gbl(r.a.out), r.a.n, gbl(r.a.skip), bx, r.grid.x);
 }
 
 // ------------------------------------------------------------------ permutes
@@ -168,7 +168,7 @@ __global__ void theta_to_svd(const c128* __restrict__ th, c128* __restrict__ X, 
 __global__ void theta_to_svd_grouped(const GRec<ThetaSvdArgs>* __restrict__ recs, int n) {
   unsigned bx, by, bz;
   const GRec<ThetaSvdArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
-  theta_to_svd_body(r.a.th, r.a.X, r.a.chil, r.a.chir, r.a.mode, bx, r.grid.x);
+  theta_to_svd_body(gbl(r.a.th), gbl(r.a.X), r.a.chil, r.a.chir, r.a.mode, bx, r.grid.x);
 }
 
 // A[p][a][k] = X[perm[k]][(p,a)] / sigma[k]   (mode 0, left isometry from U)
@@ -203,7 +203,8 @@ __global__ void gather_isometry_grouped(const GRec<GatherIsoArgs>* __restrict__ 
   unsigned bx, by, bz;
   const GRec<GatherIsoArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
   const GatherIsoArgs a = r.a;
-  gather_isometry_body(a.X, a.perm, a.sigma, a.A, a.rows, a.chi_bond, a.k, a.mode, bx, r.grid.x);
+  gather_isometry_body(gbl(a.X), gbl(a.perm), gbl(a.sigma), gbl(a.A), a.rows, a.chi_bond, a.k, a.mode, bx,
+                       r.grid.x);
 }
 
 // Row-major (rows x cols) -> column-major copy: out[c*rows + r] = in[r*cols + c]
@@ -623,7 +624,7 @@ __global__ void lz_combine(const LanczosCtrl* ctrl, const c128* __restrict__ V, 
 __global__ void lz_norm_partials_grouped(const GRec<LzArgs>* __restrict__ recs, int n) {
   unsigned bx, by, bz;
   const GRec<LzArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
-  lz_norm_partials_body(r.a.x, r.a.n, r.a.part_norm, bx);
+  lz_norm_partials_body(gbl(r.a.x), r.a.n, gbl(r.a.part_norm), bx);
 }
 __global__ void lz_init_finalize_grouped(const GRec<LzArgs>* __restrict__ recs, int n) {
   if (threadIdx.x != 0) return;
@@ -641,14 +642,15 @@ __global__ void lz_init_finalize_grouped(const GRec<LzArgs>* __restrict__ recs, 
 __global__ void lz_scale_first_grouped(const GRec<LzArgs>* __restrict__ recs, int n) {
   unsigned bx, by, bz;
   const GRec<LzArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
-  lz_scale_first_body(r.a.x, r.a.V, r.a.n, r.a.ctrl, r.a.v32, bx, r.grid.x);
+  lz_scale_first_body(gbl(r.a.x), gbl(r.a.V), r.a.n, gbl(r.a.ctrl), gbl(r.a.v32), bx, r.grid.x);
 }
 template <typename TW>
 __global__ __launch_bounds__(LZ_BLOCK) void lz_project_grouped(const GRec<LzArgs>* __restrict__ recs, int n,
                                                                int j) {
   unsigned bx, by, bz;
   const GRec<LzArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
-  lz_project_body<TW>(r.a.ctrl, reinterpret_cast<const cplx<TW>*>(r.a.w), r.a.V, r.a.n, j, r.a.part, bx);
+  lz_project_body<TW>(gbl(r.a.ctrl), gbl(reinterpret_cast<const cplx<TW>*>(r.a.w)), gbl(r.a.V), r.a.n, j,
+                      gbl(r.a.part), bx);
 }
 template <typename TW>
 __global__ __launch_bounds__(LZ_BLOCK) void lz_subtract_grouped(const GRec<LzArgs>* __restrict__ recs, int n,
@@ -657,23 +659,24 @@ __global__ __launch_bounds__(LZ_BLOCK) void lz_subtract_grouped(const GRec<LzArg
   const GRec<LzArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
   const LzArgs a = r.a;
   if (pass == 1)
-    lz_subtract_body<TW>(a.ctrl, reinterpret_cast<const cplx<TW>*>(a.w), a.w64, a.V, a.n, j, a.part,
-                         a.nblk, a.coef1, a.part2, 1, bx);
+    lz_subtract_body<TW>(gbl(a.ctrl), gbl(reinterpret_cast<const cplx<TW>*>(a.w)), gbl(a.w64), gbl(a.V), a.n,
+                         j, gbl(a.part), a.nblk, gbl(a.coef1), gbl(a.part2), 1, bx);
   else
-    lz_subtract_body<double>(a.ctrl, a.w64, a.w64, a.V, a.n, j, a.part2, a.nblk, a.coef2, a.part, 2, bx);
+    lz_subtract_body<double>(gbl(a.ctrl), gbl(a.w64), gbl(a.w64), gbl(a.V), a.n, j, gbl(a.part2), a.nblk,
+                             gbl(a.coef2), gbl(a.part), 2, bx);
 }
 __global__ __launch_bounds__(256) void lz_step_scale_grouped(const GRec<LzArgs>* __restrict__ recs, int n,
                                                              int j) {
   unsigned bx, by, bz;
   const GRec<LzArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
   const LzArgs a = r.a;
-  lz_step_scale_body(a.ctrl, a.part, a.nblk, a.coef1, a.coef2, j, a.tau, a.tol, a.max_m, a.w64,
-                     a.V + (long)(j + 1) * a.n, a.v32, a.n, bx, r.grid.x);
+  lz_step_scale_body(gbl(a.ctrl), gbl(a.part), a.nblk, gbl(a.coef1), gbl(a.coef2), j, a.tau, a.tol, a.max_m,
+                     gbl(a.w64), gbl(a.V + (long)(j + 1) * a.n), gbl(a.v32), a.n, bx, r.grid.x);
 }
 __global__ void lz_combine_grouped(const GRec<LzArgs>* __restrict__ recs, int n) {
   unsigned bx, by, bz;
   const GRec<LzArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
-  lz_combine_body(r.a.ctrl, r.a.V, r.a.n, r.a.y, bx, r.grid.x);
+  lz_combine_body(gbl(r.a.ctrl), gbl(r.a.V), r.a.n, gbl(r.a.y), bx, r.grid.x);
 }
 
 // ------------------------------------------------------------------ measurement

@@ -508,7 +508,18 @@ template <typename TE>
 __global__ __launch_bounds__(FK_THREADS) void fused_krylov_grouped(const GRec<FusedKrylovArgs<TE>>* __restrict__ recs,
                                                                    int n) {
   extern __shared__ __align__(16) unsigned char arena[];
-  const FusedKrylovArgs<TE> a = recs[blockIdx.x].a;
+  FusedKrylovArgs<TE> a = recs[blockIdx.x].a;
+  a.L = gbl(a.L);
+  a.R = gbl(a.R);
+  a.rowptr = gbl(a.rowptr);
+  a.col = gbl(a.col);
+  a.val = gbl(a.val);
+  a.x = gbl(a.x);
+  a.V = gbl(a.V);
+  a.T1 = gbl(a.T1);
+  a.U = gbl(a.U);
+  a.ctrl = gbl(a.ctrl);
+  a.tally = gbl(a.tally);
   fused_krylov_body<TE>(a, arena);
 }
 

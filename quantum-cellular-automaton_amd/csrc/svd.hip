@@ -47,6 +47,7 @@ constexpr int JAC_RC = 64;           // rows per Gram / apply chunk
 constexpr int JAC_GRAM_NT = 256;     // jac_gram: one wave per 16x16 block of the 32x32 Gram
 constexpr int JAC_APPLY_NT = 512;    // jac_apply: one wave per 16x16 block of the 32x64 product
 constexpr int JAC_MAX_SWEEPS = 60;
+constexpr int JAC_MAXKSPL = 8;       // row chunks of a Gram: rows <= 512
 // Noise floor of a column, squared and relative to ||X||_F^2: (16 eps)^2.
 // Columns below 16 eps ||X|| (3.6e-15 ||X||) are not orthogonalised further;
 // their singular values are at the accuracy floor of the method, far below any
@@ -63,6 +64,36 @@ __device__ __forceinline__ double jac_conv_tol(double tol, int rows) {
 }
 
 __device__ __forceinline__ double wsum(double v) { return wave_sum64(v); }
+
+// Sum over rows r = lane, lane + 64, ... < rows (rows <= 512) of |col[r]|^2 in
+// row order, with all of the lane's loads issued before the first add.
+__device__ __forceinline__ double lane_norm2(const c128* __restrict__ col, int rows, int lane) {
+  c128 v[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+    if (lane + 64 * i < rows) v[i] = col[lane + 64 * i];
+  double a = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+    if (lane + 64 * i < rows) a += v[i].re * v[i].re + v[i].im * v[i].im;
+  return a;
+}
+
+// Sum of |x[e]|^2 over e = t, t + stride, ... < n in that order, eight loads
+// in flight at a time.
+__device__ __forceinline__ double strided_norm2(const c128* __restrict__ x, long n, long t, long stride) {
+  double part = 0;
+  for (long e0 = t; e0 < n; e0 += 8 * stride) {
+    c128 v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (e0 + i * stride < n) v[i] = x[e0 + i * stride];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (e0 + i * stride < n) part += v[i].re * v[i].re + v[i].im * v[i].im;
+  }
+  return part;
+}
 
 // Position t of round r in the round-robin tournament over n (even) players.
 __device__ __forceinline__ int rr_player(int t, int r, int n) {
@@ -84,18 +115,27 @@ struct JacobiGeom {
 // tile with a padded row stride (JAC_LMAX + 1) that spreads the banks; rows
 // beyond nr and columns beyond 2bs (up to JAC_LMAX) are zero.  Lane bits give
 // the row, the wave index walks the columns: no divisions, and each wave reads
-// one contiguous 64-row run of a column.
-__device__ void stage_tile(const c128* __restrict__ X, const JacobiGeom& g, int p, int round,
-                           int r0, int nr, c128* tile) {
+// one contiguous 64-row run of a column.  NT is the workgroup size: all of a
+// thread's loads are issued before its first LDS store, so the tile costs one
+// memory latency, not one per column it holds.
+template <int NT>
+__device__ __forceinline__ void stage_tile(const c128* __restrict__ X, const JacobiGeom& g, int p,
+                                           int round, int r0, int nr, c128* tile) {
   static_assert(JAC_RC == 64, "one row per lane");
+  static_assert(NT % 64 == 0 && JAC_LMAX % (NT / 64) == 0, "whole columns per wave");
+  constexpr int NW = NT / 64, IT = JAC_LMAX / NW;
   const int L = 2 * g.bs;
   const int bi = rr_player(p, round, g.nb), bj = rr_player(g.nb - 1 - p, round, g.nb);
-  const int r = threadIdx.x & 63;
-  for (int c = threadIdx.x >> 6; c < JAC_LMAX; c += blockDim.x / 64) {
+  const int r = threadIdx.x & 63, w = threadIdx.x >> 6;
+  c128 v[IT];
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int c = w + i * NW;
     const int gc = c < g.bs ? bi * g.bs + c : bj * g.bs + (c - g.bs);
-    tile[r * (JAC_LMAX + 1) + c] =
-        (c < L && gc < g.cols && r < nr) ? X[(long)gc * g.rows + r0 + r] : c128{0.0, 0.0};
+    v[i] = (c < L && gc < g.cols && r < nr) ? X[(long)gc * g.rows + r0 + r] : c128{0.0, 0.0};
   }
+#pragma unroll
+  for (int i = 0; i < IT; ++i) tile[r * (JAC_LMAX + 1) + w + i * NW] = v[i];
 }
 
 // The pair products of a round on the FP64 matrix cores: v_mfma_f64_16x16x4f64
@@ -113,7 +153,7 @@ __device__ __forceinline__ void jac_init_body(const c128* __restrict__ X, long n
                                               JacobiCtrl* __restrict__ ctrl) {
   __shared__ double red[JAC_NT / 64];
   double part = 0;
-  for (long e = threadIdx.x; e < n; e += JAC_NT) part += X[e].re * X[e].re + X[e].im * X[e].im;
+  part = strided_norm2(X, n, threadIdx.x, JAC_NT);
   part = wsum(part);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
   __syncthreads();
@@ -138,12 +178,15 @@ __global__ __launch_bounds__(JAC_NT) void jac_init(const c128* __restrict__ X, l
 __device__ __forceinline__ void jac_gram_body(const c128* __restrict__ X, const JacobiGeom& g, int round,
                                               c128* __restrict__ Gpart,
                                               const JacobiCtrl* __restrict__ ctrl, int p, int kc) {
-  if (ctrl->done) return;
+  // the control word is read alongside the tile, not before it: the round
+  // kernels are latency bound, and a no-op round only stages a tile
+  const int done = ctrl->done;
   __shared__ c128 tile[JAC_RC * (JAC_LMAX + 1)];
   const int L = 2 * g.bs;
   const int r0 = kc * JAC_RC, nr = min(JAC_RC, g.rows - r0);
-  stage_tile(X, g, p, round, r0, nr, tile);
+  stage_tile<JAC_GRAM_NT>(X, g, p, round, r0, nr, tile);
   __syncthreads();
+  if (done) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int tj = (wave >> 1) * 16, ti = (wave & 1) * 16;
   if (tj >= L || ti >= L) return;
@@ -316,7 +359,7 @@ __device__ __forceinline__ void jac_inner_body(const JacobiGeom& g, int round, i
                                                const c128* __restrict__ Gpart,
                                                c128* __restrict__ Jout, int* __restrict__ rotated,
                                                double tol, JacobiCtrl* __restrict__ ctrl, int p) {
-  if (ctrl->done) return;
+  const int done = ctrl->done;  // read alongside the partial Grams
   constexpr int HM = JAC_LMAX / 2;
   __shared__ c128 G[JAC_LMAX * JAC_LMAX];
   __shared__ c128 J[JAC_LMAX * JAC_LMAX];
@@ -325,21 +368,30 @@ __device__ __forceinline__ void jac_inner_body(const JacobiGeom& g, int round, i
   const int L = 2 * g.bs, H = g.bs, tid = threadIdx.x;
   const double negl = JAC_NOISE2 * ctrl->fro;
   const double conv = jac_conv_tol(tol, g.rows);
-  for (int e = tid; e < L * L; e += JAC_NT) {
+  static_assert(JAC_NT >= JAC_LMAX * JAC_LMAX, "one Gram entry per thread");
+  if (tid < L * L) {
+    // all chunk partials are loaded before the first add (one memory latency),
+    // then summed in chunk order
+    c128 v[JAC_MAXKSPL];
+#pragma unroll
+    for (int kc = 0; kc < JAC_MAXKSPL; ++kc)
+      if (kc < g.kspl) v[kc] = Gpart[((long)p * g.kspl + kc) * L * L + tid];
     double re = 0, im = 0;
-    for (int kc = 0; kc < g.kspl; ++kc) {
-      const c128 v = Gpart[((long)p * g.kspl + kc) * L * L + e];
-      re += v.re;
-      im += v.im;
-    }
-    G[e] = c128{re, im};
-    J[e] = c128{(e % L) == (e / L) ? 1.0 : 0.0, 0.0};
+#pragma unroll
+    for (int kc = 0; kc < JAC_MAXKSPL; ++kc)
+      if (kc < g.kspl) {
+        re += v[kc].re;
+        im += v[kc].im;
+      }
+    G[tid] = c128{re, im};
+    J[tid] = c128{(tid % L) == (tid / L) ? 1.0 : 0.0, 0.0};
   }
   if (tid == 0) {
     total = 0;
     total_sig = 0;
   }
   __syncthreads();
+  if (done) return;
   const bool cross = round != 0 && round != (g.nb - 1) / 2;
   const int mine = cross ? inner_sweep<true>(G, J, rot, L, H, tol * tol, conv * conv, negl)
                          : inner_sweep<false>(G, J, rot, L, H, tol * tol, conv * conv, negl);
@@ -370,7 +422,8 @@ __device__ __forceinline__ void jac_apply_body(c128* __restrict__ Xa, const Jaco
                                                const c128* __restrict__ Jin,
                                                const int* __restrict__ rotated,
                                                const JacobiCtrl* __restrict__ ctrl, int p, int by) {
-  if (ctrl->done || rotated[p] == 0) return;
+  // the control words are read alongside the tile and J, not before them
+  const int skip = ctrl->done | (rotated[p] == 0);
   const bool second = by >= ga.kspl;
   c128* __restrict__ X = second ? Za : Xa;
   const JacobiGeom g = second ? gz : ga;
@@ -379,13 +432,22 @@ __device__ __forceinline__ void jac_apply_body(c128* __restrict__ Xa, const Jaco
   __shared__ c128 J[JAC_LMAX * (JAC_LMAX + 1)];   // J(k, c) at c * (JAC_LMAX + 1) + k, zero-padded
   const int L = 2 * g.bs;
   const int r0 = kc * JAC_RC, nr = min(JAC_RC, g.rows - r0);
-  stage_tile(X, g, p, round, r0, nr, tile);
+  stage_tile<JAC_APPLY_NT>(X, g, p, round, r0, nr, tile);
   const c128* jp = Jin + (long)p * L * L;
-  for (int e = threadIdx.x; e < JAC_LMAX * JAC_LMAX; e += blockDim.x) {
-    const int k = e % JAC_LMAX, c = e / JAC_LMAX;
-    J[c * (JAC_LMAX + 1) + k] = (k < L && c < L) ? jp[c * L + k] : c128{0.0, 0.0};
+  constexpr int JIT = JAC_LMAX * JAC_LMAX / JAC_APPLY_NT;
+  c128 jv[JIT];
+#pragma unroll
+  for (int i = 0; i < JIT; ++i) {
+    const int e = threadIdx.x + i * JAC_APPLY_NT, k = e % JAC_LMAX, c = e / JAC_LMAX;
+    jv[i] = (k < L && c < L) ? jp[c * L + k] : c128{0.0, 0.0};
+  }
+#pragma unroll
+  for (int i = 0; i < JIT; ++i) {
+    const int e = threadIdx.x + i * JAC_APPLY_NT, k = e % JAC_LMAX, c = e / JAC_LMAX;
+    J[c * (JAC_LMAX + 1) + k] = jv[i];
   }
   __syncthreads();
+  if (skip) return;
   // Eight waves: wave w computes the 16x16 block (columns c in 16(w%2)..,
   // rows r in 16(w/2)..) of the transposed product D[c][r] = sum_k J(k, c) X(r, k),
   // so that consecutive lanes store consecutive rows of a column.
@@ -439,12 +501,7 @@ __device__ void finalize_body(const c128* __restrict__ X, int rows, int cols,
   int np2 = 1;
   while (np2 < cols) np2 <<= 1;
   for (int c = wave; c < np2; c += nw) {
-    double a = 0;
-    if (c < cols)
-      for (int r = lane; r < rows; r += 64) {
-        const c128 v = X[(long)c * rows + r];
-        a += v.re * v.re + v.im * v.im;
-      }
+    double a = (c < cols) ? lane_norm2(X + (long)c * rows, rows, lane) : 0.0;
     a = wsum(a);
     if (lane == 0) {
       skey[c] = (c < cols) ? sqrt(a) : -1.0;
@@ -525,7 +582,7 @@ __device__ __forceinline__ void jac_small_body(c128* __restrict__ X, int rows, i
   const int H = (cols + 1) / 2, L = 2 * H;
   const double conv = jac_conv_tol(tol, rows);
   double part = 0;
-  for (long e = tid; e < (long)rows * cols; e += JAC_NT) part += X[e].re * X[e].re + X[e].im * X[e].im;
+  part = strided_norm2(X, (long)rows * cols, tid, JAC_NT);
   part = wsum(part);
   if ((tid & 63) == 0) red[tid >> 6] = part;
   __syncthreads();
@@ -541,7 +598,7 @@ __device__ __forceinline__ void jac_small_body(c128* __restrict__ X, int rows, i
     for (int r0 = 0; r0 < rows; r0 += JAC_RC) {
       const int nr = min(JAC_RC, rows - r0);
       __syncthreads();
-      stage_tile(X, g, 0, 0, r0, nr, tile);
+      stage_tile<JAC_NT>(X, g, 0, 0, r0, nr, tile);
       __syncthreads();
       if (tid < L * L)
         for (int r = 0; r < nr; ++r) {
@@ -570,7 +627,7 @@ __device__ __forceinline__ void jac_small_body(c128* __restrict__ X, int rows, i
     // X <- X J, chunk by chunk (each chunk is read into the tile before it is written)
     for (int r0 = 0; r0 < rows; r0 += JAC_RC) {
       const int nr = min(JAC_RC, rows - r0);
-      stage_tile(X, g, 0, 0, r0, nr, tile);
+      stage_tile<JAC_NT>(X, g, 0, 0, r0, nr, tile);
       __syncthreads();
       const int r = tid & 63;
       if (r < nr)
@@ -739,9 +796,20 @@ __device__ __forceinline__ void qr_panel_body(c128* __restrict__ M, int rows, in
   __shared__ double s_xnorm2;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nr = rows - j0;
-  for (int e = tid; e < pb * nr; e += QR_THREADS) {
-    const int c = e / nr, r = e - c * nr;
-    P[c][r] = M[(long)(j0 + c) * rows + j0 + r];
+  {
+    // wave w stages column w, lane l rows l, l + 64, ...: all of a thread's
+    // loads are issued before its first LDS store.  Slots past the panel
+    // (columns >= pb, rows >= nr) load a clamped valid element and are never
+    // read, so neither loads nor stores branch.
+    static_assert(QR_WAVES == QRB_PB && QRB_MAXROWS % 64 == 0, "one wave per panel column");
+    constexpr int IT = QRB_MAXROWS / 64;
+    const int c = min(wave, pb - 1);
+    const c128* src = M + (long)(j0 + c) * rows + j0;
+    c128 v[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) v[i] = src[min(lane + 64 * i, nr - 1)];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) P[wave][lane + 64 * i] = v[i];
   }
   __syncthreads();
   for (int c = 0; c < pb; ++c) {
@@ -850,7 +918,9 @@ __device__ __forceinline__ void qr_panel_body(c128* __restrict__ M, int rows, in
     }
   }
 }
-__global__ __launch_bounds__(QR_THREADS) void qr_panel_kernel(c128* __restrict__ M, int rows, int j0,
+// One workgroup per CU (the panel fills LDS): a register budget for 4 waves per
+// SIMD lets the panel load keep all its loads in flight.
+__global__ __launch_bounds__(QR_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4))) void qr_panel_kernel(c128* __restrict__ M, int rows, int j0,
                                                               int pb, c128* __restrict__ tau,
                                                               c128* __restrict__ V,
                                                               c128* __restrict__ T) {
@@ -879,12 +949,7 @@ __device__ __forceinline__ void col_norm_order_body(const c128* __restrict__ X, 
   int np2 = 1;
   while (np2 < cols) np2 <<= 1;
   for (int c = wave; c < np2; c += nw) {
-    double a = 0;
-    if (c < cols)
-      for (int r = lane; r < rows; r += 64) {
-        const c128 v = X[(long)c * rows + r];
-        a += v.re * v.re + v.im * v.im;
-      }
+    double a = (c < cols) ? lane_norm2(X + (long)c * rows, rows, lane) : 0.0;
     a = wsum(a);
     if (lane == 0) {
       skey[c] = (c < cols) ? a : -1.0;
@@ -956,12 +1021,7 @@ __device__ __forceinline__ void scale_by_norms_body(const c128* __restrict__ Y, 
                                                     c128* __restrict__ X, unsigned blk, unsigned nblk) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int j = blk * (JAC_NT / 64) + wave; j < kq; j += nblk * (JAC_NT / 64)) {
-    double a = 0;
-    for (int r = lane; r < yrows; r += 64) {
-      const c128 v = Y[(long)j * yrows + r];
-      a += v.re * v.re + v.im * v.im;
-    }
-    const double sg = sqrt(wsum(a));
+    const double sg = sqrt(wsum(lane_norm2(Y + (long)j * yrows, yrows, lane)));
     for (int r = lane; r < rows; r += 64) {
       const c128 z = Z[(long)j * rows + r];
       X[(long)j * rows + r] = c128{sg * z.re, sg * z.im};
@@ -1030,40 +1090,41 @@ struct FinArgs {
 
 __global__ __launch_bounds__(JAC_NT) void jac_small_grouped(const GRec<JacSmallArgs>* __restrict__ recs, int n) {
   const JacSmallArgs a = recs[blockIdx.x].a;
-  jac_small_body(a.X, a.rows, a.cols, a.sigma, a.perm, a.res, a.chi_max, a.eps, a.rank_cap, a.tol);
+  jac_small_body(gbl(a.X), a.rows, a.cols, gbl(a.sigma), gbl(a.perm), gbl(a.res), a.chi_max, a.eps, a.rank_cap,
+                 a.tol);
 }
 __global__ __launch_bounds__(JAC_NT) void col_norm_order_grouped(const GRec<ColOrderArgs>* __restrict__ recs,
                                                                  int n) {
   const ColOrderArgs a = recs[blockIdx.x].a;
-  col_norm_order_body(a.X, a.rows, a.cols, a.order);
+  col_norm_order_body(gbl(a.X), a.rows, a.cols, gbl(a.order));
 }
 __global__ void permute_cols_grouped(const GRec<ColOrderArgs>* __restrict__ recs, int n) {
   unsigned bx, by, bz;
   const GRec<ColOrderArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
-  permute_cols_body(r.a.X, r.a.rows, r.a.cols, r.a.order, r.a.Xp, bx, r.grid.x);
+  permute_cols_body(gbl(r.a.X), r.a.rows, r.a.cols, gbl(r.a.order), gbl(r.a.Xp), bx, r.grid.x);
 }
-__global__ __launch_bounds__(QR_THREADS) void qr_panel_grouped(const GRec<QrPanelArgs>* __restrict__ recs, int n) {
+__global__ __launch_bounds__(QR_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4))) void qr_panel_grouped(const GRec<QrPanelArgs>* __restrict__ recs, int n) {
   const QrPanelArgs a = recs[blockIdx.x].a;
-  qr_panel_body(a.M, a.rows, a.j0, a.pb, a.tau, a.V, a.T);
+  qr_panel_body(gbl(a.M), a.rows, a.j0, a.pb, gbl(a.tau), gbl(a.V), gbl(a.T));
 }
 __global__ void set_identity_grouped(const GRec<QrIdentArgs>* __restrict__ recs, int n) {
   unsigned bx, by, bz;
   const GRec<QrIdentArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
-  set_identity_body(r.a.Q, r.a.rows, r.a.cols, bx, r.grid.x);
+  set_identity_body(gbl(r.a.Q), r.a.rows, r.a.cols, bx, r.grid.x);
 }
 __global__ void rh_from_qr_grouped(const GRec<RhArgs>* __restrict__ recs, int n) {
   unsigned bx, by, bz;
   const GRec<RhArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
-  rh_from_qr_body(r.a.M, r.a.rows, r.a.cols, r.a.kq, r.a.Y, bx, r.grid.x);
+  rh_from_qr_body(gbl(r.a.M), r.a.rows, r.a.cols, r.a.kq, gbl(r.a.Y), bx, r.grid.x);
 }
 __global__ __launch_bounds__(JAC_NT) void scale_by_norms_grouped(const GRec<ScaleArgs>* __restrict__ recs, int n) {
   unsigned bx, by, bz;
   const GRec<ScaleArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
-  scale_by_norms_body(r.a.Y, r.a.yrows, r.a.Z, r.a.rows, r.a.kq, r.a.X, bx, r.grid.x);
+  scale_by_norms_body(gbl(r.a.Y), r.a.yrows, gbl(r.a.Z), r.a.rows, r.a.kq, gbl(r.a.X), bx, r.grid.x);
 }
 __global__ __launch_bounds__(JAC_NT) void jac_init_grouped(const GRec<JacArgs>* __restrict__ recs, int n) {
   const JacArgs& a = recs[blockIdx.x].a;
-  jac_init_body(a.Y, (long)a.g.rows * a.g.cols, a.ctrl);
+  jac_init_body(gbl(a.Y), (long)a.g.rows * a.g.cols, gbl(a.ctrl));
 }
 // Rounds beyond a matrix's own tournament (nb - 1 rounds) are no-ops for it.
 __global__ __launch_bounds__(JAC_GRAM_NT) void jac_gram_grouped(const GRec<JacArgs>* __restrict__ recs, int n,
@@ -1072,7 +1133,7 @@ __global__ __launch_bounds__(JAC_GRAM_NT) void jac_gram_grouped(const GRec<JacAr
   const GRec<JacArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
   const JacobiGeom g = r.a.g;
   if (round >= g.nb - 1) return;
-  jac_gram_body(r.a.Y, g, round, r.a.Gpart, r.a.ctrl, bx, by);
+  jac_gram_body(gbl(r.a.Y), g, round, gbl(r.a.Gpart), gbl(r.a.ctrl), bx, by);
 }
 __global__ __launch_bounds__(JAC_NT) __attribute__((amdgpu_waves_per_eu(8))) void jac_inner_grouped(const GRec<JacArgs>* __restrict__ recs, int n,
                                                             int round, int sweep) {
@@ -1080,7 +1141,7 @@ __global__ __launch_bounds__(JAC_NT) __attribute__((amdgpu_waves_per_eu(8))) voi
   const GRec<JacArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
   const JacobiGeom g = r.a.g;
   if (round >= g.nb - 1) return;
-  jac_inner_body(g, round, sweep, r.a.Gpart, r.a.Jbuf, r.a.rotated, r.a.tol, r.a.ctrl, bx);
+  jac_inner_body(g, round, sweep, gbl(r.a.Gpart), gbl(r.a.Jbuf), gbl(r.a.rotated), r.a.tol, gbl(r.a.ctrl), bx);
 }
 __global__ __launch_bounds__(JAC_APPLY_NT) void jac_apply_grouped(const GRec<JacArgs>* __restrict__ recs, int n,
                                                             int round) {
@@ -1088,7 +1149,7 @@ __global__ __launch_bounds__(JAC_APPLY_NT) void jac_apply_grouped(const GRec<Jac
   const GRec<JacArgs>& r = grec_find(recs, n, blockIdx.x, bx, by, bz);
   const JacobiGeom g = r.a.g, gz = r.a.gz;
   if (round >= g.nb - 1) return;
-  jac_apply_body(r.a.Y, g, r.a.Z, gz, round, r.a.Jbuf, r.a.rotated, r.a.ctrl, bx, by);
+  jac_apply_body(gbl(r.a.Y), g, gbl(r.a.Z), gz, round, gbl(r.a.Jbuf), gbl(r.a.rotated), gbl(r.a.ctrl), bx, by);
 }
 __global__ void jac_check_grouped(const GRec<JacArgs>* __restrict__ recs, int n, int sweep) {
   JacobiCtrl* ctrl = recs[blockIdx.x].a.ctrl;
@@ -1103,8 +1164,9 @@ __global__ void jac_done_gather(const GRec<JacArgs>* __restrict__ recs, int n, i
 }
 __global__ __launch_bounds__(JAC_NT) void jac_finalize_grouped(const GRec<FinArgs>* __restrict__ recs, int n) {
   const FinArgs a = recs[blockIdx.x].a;
-  finalize_body(a.X, a.rows, a.cols, a.sigma, a.perm, a.res, a.chi_max, a.eps, a.rank_cap, a.ctrl->sweeps,
-                a.ctrl->done);
+  const JacobiCtrl* ctrl = gbl(a.ctrl);
+  finalize_body(gbl(a.X), a.rows, a.cols, gbl(a.sigma), gbl(a.perm), gbl(a.res), a.chi_max, a.eps, a.rank_cap,
+                ctrl->sweeps, ctrl->done);
 }
 
 }  // namespace
@@ -1115,6 +1177,7 @@ JacobiLayout jacobi_layout(int rows, int cols) {
   l.nb = std::max(2, ceil_div(cols, l.bs));
   if (l.nb & 1) ++l.nb;
   l.kspl = ceil_div(rows, JAC_RC);
+  QCA_CHECK(l.kspl <= JAC_MAXKSPL, "Jacobi rows out of range (1..512)");
   const long L = 2L * l.bs, np = l.nb / 2;
   l.gpart_elems = np * l.kspl * L * L;
   l.j_elems = np * L * L;
