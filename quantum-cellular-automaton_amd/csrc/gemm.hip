@@ -93,7 +93,9 @@ __device__ __forceinline__ void gemm_body(const GemmDesc& d, const cplx<T>* __re
   constexpr int STAGE = 2 * PA::SIZE + 2 * PB::SIZE;  // [A re, A im, B re, B im]
   __shared__ T smem[KG * 2 * STAGE];
   static_assert(KG == 1 || KG * 2 * STAGE >= 4 * 32 * 64, "partial-tile exchange fits the operand LDS");
-  if (d.skip && *d.skip) return;
+  // the skip word is read here and tested once the first tile's loads are in
+  // flight: the two memory latencies overlap (a skipped launch only loads)
+  const int skipped = d.skip ? *d.skip : 0;
 
   const int tid = threadIdx.x & 255;  // thread within its K-group
   // the K-group is wave-uniform: keeping it (and so the tile schedule, the
@@ -235,6 +237,7 @@ __device__ __forceinline__ void gemm_body(const GemmDesc& d, const cplx<T>* __re
   T* const xs = smem + (wave * 32) * 64 + lane;  // partial-tile exchange: [wave][value][lane]
 
   load_tile(ch_begin < ch_end ? tile_at(ch_begin, 0) : -1);
+  if (skipped) return;
   int it = 0;  // iterations so far: the LDS stage parity
   for (int ch = ch_begin; ch < ch_end; ++ch) {
     const int nit = chunk_iters(ch);
@@ -411,7 +414,7 @@ template <typename T>
 __device__ __forceinline__ void splitk_reduce_body(const GemmDesc& d, const cplx<T>* __restrict__ ws,
                                                    cplx<T>* __restrict__ C, int splits, int batch,
                                                    unsigned blk, unsigned nblk) {
-  if (d.skip && *d.skip) return;
+  const int skipped = d.skip ? *d.skip : 0;  // tested after the first partial loads
   const long mn = (long)d.M * d.N;
   const long total = mn * batch;
   for (long idx = blk * (long)blockDim.x + threadIdx.x; idx < total;
@@ -428,6 +431,7 @@ __device__ __forceinline__ void splitk_reduce_body(const GemmDesc& d, const cplx
       xr += v.re;
       xi += v.im;
     }
+    if (skipped) return;
     const int b1 = zb / d.NB2, b2 = zb - b1 * d.NB2;
     cplx<T>* cp = C + b1 * d.c_b1 + b2 * d.c_b2 + m * d.c_m + n * d.c_n;
     const T al_re = T(d.alpha_re), al_im = T(d.alpha_im);

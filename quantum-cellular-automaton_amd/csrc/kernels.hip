@@ -357,7 +357,7 @@ template <typename TW>
 __device__ __forceinline__ void lz_project_body(const LanczosCtrl* ctrl, const cplx<TW>* __restrict__ w,
                                                 const c128* __restrict__ V, long n, int j,
                                                 c128* __restrict__ part, unsigned blk) {
-  if (ctrl->done) return;
+  const int done = ctrl->done;  // tested once w's loads are in flight
   const long base = (long)blk * LZ_ELEMS + threadIdx.x;
   double wr[LZ_EPT], wi[LZ_EPT];
 #pragma unroll
@@ -366,6 +366,7 @@ __device__ __forceinline__ void lz_project_body(const LanczosCtrl* ctrl, const c
     wr[r] = e < n ? double(w[e].re) : 0.0;
     wi[r] = e < n ? double(w[e].im) : 0.0;
   }
+  if (done) return;
   block_projections(V, n, j, base, wr, wi, part + (long)blk * LZ_MAXK);
 }
 template <typename TW>
@@ -389,7 +390,7 @@ __device__ __forceinline__ void lz_subtract_body(const LanczosCtrl* ctrl, const 
                                                  int nblk, c128* __restrict__ coef_out,
                                                  c128* __restrict__ part_out, int pass, unsigned blk,
                                                  const c128* __restrict__ coef_in = nullptr) {
-  if (ctrl->done) return;
+  const int done = ctrl->done;  // tested after the coefficient loads
   __shared__ double sh[16];
   __shared__ c128 cs[LZ_MAXK];
   if (coef_in) {
@@ -407,11 +408,12 @@ __device__ __forceinline__ void lz_subtract_body(const LanczosCtrl* ctrl, const 
       ai = wave_sum(ai);
       if (lane == 0) {
         cs[i] = c128{ar, ai};
-        if (blk == 0) coef_out[i] = c128{ar, ai};
+        if (blk == 0 && !done) coef_out[i] = c128{ar, ai};
       }
     }
   }
   __syncthreads();
+  if (done) return;
   const long base = (long)blk * LZ_ELEMS + threadIdx.x;
   double xr[LZ_EPT], xi[LZ_EPT];
 #pragma unroll
@@ -566,13 +568,14 @@ __device__ __forceinline__ void lz_step_scale_body(LanczosCtrl* ctrl, const c128
                                                    c128* __restrict__ vnext, c64* __restrict__ v32,
                                                    long n, unsigned blk, unsigned ngrid,
                                                    int beta_reduced = 0) {
-  if (ctrl->done) return;
+  const int done = ctrl->done;  // tested after beta's loads
   __shared__ double sbeta;
   if (threadIdx.x < 64) {
     const double b = beta_reduced ? sqrt(ctrl->wnorm2) : lz_beta(part_norm, nblk);
     if (threadIdx.x == 0) sbeta = b;
   }
   __syncthreads();
+  if (done) return;
   const double beta = sbeta;
   if (blk == 0 && threadIdx.x < 64) lz_close_iteration(ctrl, beta, coef1, coef2, j, tau, tol, max_m);
   const double inv = 1.0 / beta;
