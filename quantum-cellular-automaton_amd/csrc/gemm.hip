@@ -423,13 +423,21 @@ __device__ __forceinline__ void splitk_reduce_body(const GemmDesc& d, const cplx
     const long r = idx - zb * mn;
     const int m = (int)(r / d.N), n = (int)(r - (long)m * d.N);
     T xr = 0, xi = 0;
-    // unrolled so that several partial loads are in flight; the sum stays in
-    // split order
-#pragma unroll 8
-    for (int s = 0; s < splits; ++s) {
-      const cplx<T> v = ws[((long)(zb * splits + s) * d.M + m) * d.N + n];
-      xr += v.re;
-      xi += v.im;
+    // 16 partial loads in flight per block of splits (one memory latency for
+    // the 15 splits of the two-site chain): slots past the last split load it
+    // again and add nothing; the sum stays in split order
+    const cplx<T>* wp = ws + ((long)zb * splits * d.M + m) * d.N + n;
+    const long sstride = (long)d.M * d.N;
+    for (int s0 = 0; s0 < splits; s0 += 16) {
+      cplx<T> v[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) v[i] = wp[min(s0 + i, splits - 1) * sstride];
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+        if (s0 + i < splits) {
+          xr += v[i].re;
+          xi += v[i].im;
+        }
     }
     if (skipped) return;
     const int b1 = zb / d.NB2, b2 = zb - b1 * d.NB2;

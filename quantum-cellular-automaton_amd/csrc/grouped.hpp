@@ -160,6 +160,7 @@ struct QrPanelArgs {
   c128* tau;
   c128* V;
   c128* T;
+  c128* VT;   // V T of the panel, laid out like V
 };
 struct QrIdentArgs {
   c128* Q;

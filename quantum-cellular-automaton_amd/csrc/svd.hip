@@ -789,10 +789,11 @@ constexpr int QRB_MAXROWS = 512;
 
 __device__ __forceinline__ void qr_panel_body(c128* __restrict__ M, int rows, int j0, int pb,
                                               c128* __restrict__ tau, c128* __restrict__ V,
-                                              c128* __restrict__ T) {
+                                              c128* __restrict__ T, c128* __restrict__ VT) {
   __shared__ c128 P[QRB_PB][QRB_MAXROWS];   // panel, local row r = global row j0 + r
   __shared__ c128 s_tau[QRB_PB];
   __shared__ c128 s_vv[QRB_PB][QRB_PB];     // V^H V of the panel
+  __shared__ c128 s_T[QRB_PB][QRB_PB];      // s_T[i][k] = T(k, i)
   __shared__ double s_xnorm2;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nr = rows - j0;
@@ -906,7 +907,7 @@ __device__ __forceinline__ void qr_panel_body(c128* __restrict__ M, int rows, in
       if (k < i) {
         double ar = 0, ai = 0;
         for (int l = k; l < i; ++l) {
-          const c128 t = T[(long)l * QRB_PB + k], w = s_vv[l][i];
+          const c128 t = s_T[l][k], w = s_vv[l][i];
           ar += t.re * w.re - t.im * w.im;
           ai += t.re * w.im + t.im * w.re;
         }
@@ -914,8 +915,29 @@ __device__ __forceinline__ void qr_panel_body(c128* __restrict__ M, int rows, in
       } else if (k == i) {
         out = ti;
       }
-      T[(long)i * QRB_PB + k] = out;
+      s_T[i][k] = out;
     }
+  }
+  __syncthreads();
+  if (tid < pb * QRB_PB) {
+    const int i = tid / QRB_PB, k = tid - i * QRB_PB;
+    T[(long)i * QRB_PB + k] = k < pb ? s_T[i][k] : c128{0.0, 0.0};
+  }
+  // VT = V T on the panel's rows (both triangular: V(r, k) = 0 for r < k,
+  // T(k, c) = 0 for k > c), stored like V.  The trailing update uses
+  // A -= V ((V T)^H A) and the accumulation of Q uses Q -= (V T)(V^H Q), so
+  // neither needs a separate product with T.
+  for (int e = tid; e < nr * pb; e += QR_THREADS) {
+    const int c = e / nr, r = e - c * nr;
+    double ar = 0, ai = 0;
+    const int kmax = min(c, r);
+    for (int k = 0; k <= kmax; ++k) {
+      const c128 v = (r == k) ? c128{1.0, 0.0} : P[k][r];
+      const c128 t = s_T[c][k];
+      ar += v.re * t.re - v.im * t.im;
+      ai += v.re * t.im + v.im * t.re;
+    }
+    VT[(long)(j0 + c) * rows + j0 + r] = c128{ar, ai};
   }
 }
 // One workgroup per CU (the panel fills LDS): a register budget for 4 waves per
@@ -923,8 +945,9 @@ __device__ __forceinline__ void qr_panel_body(c128* __restrict__ M, int rows, in
 __global__ __launch_bounds__(QR_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4))) void qr_panel_kernel(c128* __restrict__ M, int rows, int j0,
                                                               int pb, c128* __restrict__ tau,
                                                               c128* __restrict__ V,
-                                                              c128* __restrict__ T) {
-  qr_panel_body(M, rows, j0, pb, tau, V, T);
+                                                              c128* __restrict__ T,
+                                                              c128* __restrict__ VT) {
+  qr_panel_body(M, rows, j0, pb, tau, V, T, VT);
 }
 
 __device__ __forceinline__ void set_identity_body(c128* __restrict__ Q, int rows, int cols, unsigned blk,
@@ -1105,7 +1128,7 @@ __global__ void permute_cols_grouped(const GRec<ColOrderArgs>* __restrict__ recs
 }
 __global__ __launch_bounds__(QR_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4))) void qr_panel_grouped(const GRec<QrPanelArgs>* __restrict__ recs, int n) {
   const QrPanelArgs a = recs[blockIdx.x].a;
-  qr_panel_body(gbl(a.M), a.rows, a.j0, a.pb, gbl(a.tau), gbl(a.V), gbl(a.T));
+  qr_panel_body(gbl(a.M), a.rows, a.j0, a.pb, gbl(a.tau), gbl(a.V), gbl(a.T), gbl(a.VT));
 }
 __global__ void set_identity_grouped(const GRec<QrIdentArgs>* __restrict__ recs, int n) {
   unsigned bx, by, bz;
@@ -1298,9 +1321,9 @@ void launch_jacobi_svd(c128* X, int rows, int cols, double* sigma, int* perm, Sv
 size_t qr_workspace_elems(int max_rows, int max_cols, int max_qcols) {
   const long w = std::max(max_cols, max_qcols);
   const long panels = ceil_div(std::max(1, std::min(max_rows, max_cols)), QRB_PB);
-  return (size_t)max_rows * max_cols                 // V
+  return 2L * max_rows * max_cols                    // V, V T
          + panels * QRB_PB * QRB_PB                  // T per panel
-         + 2L * QRB_PB * w                           // W, W2
+         + QRB_PB * w                                // W
          + 32L * QRB_PB * w + 4096;                  // GEMM split-K partials
 }
 
@@ -1316,10 +1339,10 @@ std::vector<QrStep> blocked_qr_steps(c128* M, int rows, int cols, c128* tau, c12
   const long panels = ceil_div(kmax, QRB_PB);
   const long wcols = std::max(cols, qcols);
   c128* V = ws;
-  c128* T = V + (long)rows * cols;
+  c128* VT = V + (long)rows * cols;
+  c128* T = VT + (long)rows * cols;
   c128* W = T + panels * QRB_PB * QRB_PB;
-  c128* W2 = W + QRB_PB * wcols;
-  c128* gws = W2 + QRB_PB * wcols;
+  c128* gws = W + QRB_PB * wcols;
   const long used = gws - ws;
   QCA_CHECK((long)ws_elems > used + 1024, "QR workspace too small");
   const long gws_elems = (long)ws_elems - used;
@@ -1338,33 +1361,27 @@ std::vector<QrStep> blocked_qr_steps(c128* M, int rows, int cols, c128* tau, c12
     QrStep st{};
     st.slot = QR_SLOT_FACTOR + 4 * p;
     st.kind = 0;
-    st.panel = QrPanelArgs{M, rows, j0, pb, tau, V, Tp};
+    st.panel = QrPanelArgs{M, rows, j0, pb, tau, V, Tp, VT};
     out.push_back(st);
     const int nt = cols - (j0 + pb), nr = rows - j0;
     if (nt <= 0) continue;
     const c128* Vp = V + (long)j0 * rows + j0;
+    const c128* VTp = VT + (long)j0 * rows + j0;
     c128* At = M + (long)(j0 + pb) * rows + j0;
-    // W = V^H A_trail  (pb x nt)
+    // W = (V T)^H A_trail = T^H V^H A_trail  (pb x nt)
     GemmDesc g{};
     g.M = pb; g.N = nt; g.K = nr; g.K2 = nr; g.NB1 = g.NB2 = 1; g.alpha_re = 1.0;
     g.a_m = rows; g.a_k2 = 1; g.conj_a = 1;
     g.b_k2 = 1; g.b_n = rows;
     g.c_m = 1; g.c_n = pb;
-    gemm_step(QR_SLOT_FACTOR + 4 * p + 1, g, Vp, At, W);
-    // W2 = T^H W
-    GemmDesc h{};
-    h.M = pb; h.N = nt; h.K = pb; h.K2 = pb; h.NB1 = h.NB2 = 1; h.alpha_re = 1.0;
-    h.a_m = QRB_PB; h.a_k2 = 1; h.conj_a = 1;
-    h.b_k2 = 1; h.b_n = pb;
-    h.c_m = 1; h.c_n = pb;
-    gemm_step(QR_SLOT_FACTOR + 4 * p + 2, h, Tp, W, W2);
-    // A_trail -= V W2
+    gemm_step(QR_SLOT_FACTOR + 4 * p + 1, g, VTp, At, W);
+    // A_trail -= V W
     GemmDesc u{};
     u.M = nr; u.N = nt; u.K = pb; u.K2 = pb; u.NB1 = u.NB2 = 1; u.alpha_re = -1.0; u.accumulate = 1;
     u.a_m = 1; u.a_k2 = rows;
     u.b_k2 = 1; u.b_n = pb;
     u.c_m = 1; u.c_n = rows;
-    gemm_step(QR_SLOT_FACTOR + 4 * p + 3, u, Vp, W2, At);
+    gemm_step(QR_SLOT_FACTOR + 4 * p + 3, u, Vp, W, At);
   }
   if (Q == nullptr || qcols == 0) return out;
   // Q = H_0 ... H_{kk-1} [e_0 .. e_{qcols-1}], block reflectors applied from
@@ -1381,27 +1398,22 @@ std::vector<QrStep> blocked_qr_steps(c128* M, int rows, int cols, c128* tau, c12
     const int j0 = p * QRB_PB, pb = std::min(QRB_PB, kk - j0);
     const int nr = rows - j0, nq = qcols - j0;
     const c128* Vp = V + (long)j0 * rows + j0;
-    const c128* Tp = T + (long)p * QRB_PB * QRB_PB;
+    const c128* VTp = VT + (long)j0 * rows + j0;
     c128* Qs = Q + (long)j0 * rows + j0;
     const int base = QR_SLOT_Q + 3 * (QR_MAX_PANELS - 1 - p);
+    // W = V^H Q(j0:, j0:), then Q(j0:, j0:) -= (V T) W
     GemmDesc g{};
     g.M = pb; g.N = nq; g.K = nr; g.K2 = nr; g.NB1 = g.NB2 = 1; g.alpha_re = 1.0;
     g.a_m = rows; g.a_k2 = 1; g.conj_a = 1;
     g.b_k2 = 1; g.b_n = rows;
     g.c_m = 1; g.c_n = pb;
     gemm_step(base, g, Vp, Qs, W);
-    GemmDesc h{};
-    h.M = pb; h.N = nq; h.K = pb; h.K2 = pb; h.NB1 = h.NB2 = 1; h.alpha_re = 1.0;
-    h.a_m = 1; h.a_k2 = QRB_PB;
-    h.b_k2 = 1; h.b_n = pb;
-    h.c_m = 1; h.c_n = pb;
-    gemm_step(base + 1, h, Tp, W, W2);
     GemmDesc u{};
     u.M = nr; u.N = nq; u.K = pb; u.K2 = pb; u.NB1 = u.NB2 = 1; u.alpha_re = -1.0; u.accumulate = 1;
     u.a_m = 1; u.a_k2 = rows;
     u.b_k2 = 1; u.b_n = pb;
     u.c_m = 1; u.c_n = rows;
-    gemm_step(base + 2, u, Vp, W2, Qs);
+    gemm_step(base + 2, u, VTp, W, Qs);
   }
   return out;
 }
@@ -1412,7 +1424,7 @@ void launch_blocked_qr(c128* M, int rows, int cols, c128* tau, c128* Q, int qcol
     if (st.kind == 0) {
       const QrPanelArgs& a = st.panel;
       hipLaunchKernelGGL(qr_panel_kernel, dim3(1), dim3(QR_THREADS), 0, s, a.M, a.rows, a.j0, a.pb, a.tau,
-                         a.V, a.T);
+                         a.V, a.T, a.VT);
     } else if (st.kind == 1) {
       gemm_c128(st.gemm.d, st.gemm.A, st.gemm.B, st.gemm.C, s, st.gemm.ws, st.gemm.ws_elems);
     } else {
