@@ -852,6 +852,7 @@ void heff_noalloc(int kind, const void* x, void* y, const void* L, const void* R
     w.nnz = (int)hs.col.size();
     w.group_rows = hs.group_rows;
     w.group_max = hs.group_max;
+    qca::set_group_starts(w, hs.rowptr, hs.group_rows);
     std::vector<qca::cplx<T>> val(hs.val.size());
     for (size_t z = 0; z < val.size(); ++z) val[z] = qca::cplx<T>{(T)hs.val[z].re, (T)hs.val[z].im};
     // pageable sources: the copies are complete when the calls return

@@ -82,6 +82,10 @@ ContractPlan<T> heff_plan(int kind, const cplx<T>* x, cplx<T>* y, const cplx<T>*
   h.seg_col = W->col;
   h.seg_outv = W->outv;
   h.seg_val = W->val;
+  if (W->ngroups == dq) {
+    h.seg_starts = 1;
+    for (int g = 0; g <= dq; ++g) h.seg_start[g] = W->group_start[g];
+  }
   h.a_m = 1; h.a_k1 = lr; h.a_k2 = chil;
   h.b_k1 = chir; h.b_k2 = (long)Dout * chir; h.b_n = 1;
   h.c_m = chir; h.c_n = 1; h.c_b2 = lr;
@@ -277,6 +281,8 @@ void WSparsePair::upload(const HostSparse& h, int nin) {
   d32.outv = d64.outv;
   d64.group_rows = d32.group_rows = h.group_rows;
   d64.group_max = d32.group_max = h.group_max;
+  set_group_starts(d64, h.rowptr, h.group_rows);
+  set_group_starts(d32, h.rowptr, h.group_rows);
   d64.nout = d32.nout = nout;
   d64.nin = d32.nin = nin;
   d64.nnz = d32.nnz = nnz;

@@ -120,7 +120,18 @@ __device__ __forceinline__ void gemm_body(const GemmDesc& d, const cplx<T>* __re
 
   const bool gather = d.seg_rows > 0;
   int e0 = 0, nseg;
-  if (gather) {
+  if (gather && d.seg_starts) {
+    // from the arguments (a select chain, no indexed copy of the descriptor)
+    int lo = d.seg_start[0], hi = d.seg_start[1];
+#pragma unroll
+    for (int g = 1; g < SEG_STARTS_MAX; ++g)
+      if (b2 == g) {
+        lo = d.seg_start[g];
+        hi = d.seg_start[g + 1];
+      }
+    e0 = lo;
+    nseg = hi - lo;
+  } else if (gather) {
     e0 = d.seg_rowptr[b2 * d.seg_rows];
     nseg = d.seg_rowptr[(b2 + 1) * d.seg_rows] - e0;
   } else {

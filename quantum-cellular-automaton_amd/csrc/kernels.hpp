@@ -3,6 +3,8 @@
 
 #include "qca_common.hpp"
 
+#include <vector>
+
 namespace qca {
 
 // Maximum Krylov dimension (one wave holds the tridiagonal matrix).
@@ -49,7 +51,18 @@ struct WSparseDev {
   int nout = 0, nin = 0, nnz = 0;
   int group_rows = 0;    // rows per output physical index (the output MPO dimension)
   int group_max = 0;     // most nonzeros in any group of group_rows rows
+  int ngroups = 0;       // nout / group_rows when at most SEG_STARTS_MAX, else 0
+  int group_start[SEG_STARTS_MAX + 1] = {};  // rowptr[g * group_rows], g <= ngroups
 };
+// Fills ngroups and group_start of w from the host row pointer.
+template <typename T>
+inline void set_group_starts(WSparseDev<T>& w, const std::vector<int>& rowptr, int group_rows) {
+  const int nout = (int)rowptr.size() - 1;
+  w.ngroups = 0;
+  if (group_rows <= 0 || nout % group_rows != 0 || nout / group_rows > SEG_STARTS_MAX) return;
+  w.ngroups = nout / group_rows;
+  for (int g = 0; g <= w.ngroups; ++g) w.group_start[g] = rowptr[(size_t)g * group_rows];
+}
 
 template <typename T>
 void launch_wmix(const cplx<T>* Tin, long in_stride, long nxc, int nin, cplx<T>* U, int nq,

@@ -65,6 +65,7 @@ inline void ensure_dynamic_lds(const void* fn, size_t bytes) {
 }
 
 // ---------------------------------------------------------------- GEMM descriptor
+constexpr int SEG_STARTS_MAX = 4;  // two-site blocks: 4 physical index pairs
 // C[b](m,n) = alpha * sum_k opA(A)[b](m,k) * opB(B)[b](k,n) + beta * C[b](m,n)
 // with every index split into at most two levels so that contractions over
 // composite indices (e.g. (MPO bond, bond) or (physical, bond)) map onto one
@@ -93,6 +94,11 @@ struct GemmDesc {
   const int* seg_col;
   const int* seg_outv;
   const void* seg_val;  // cplx<T> of the GEMM's precision
+  // seg_start[b2] = seg_rowptr[b2 * seg_rows] for b2 <= NB2 when seg_starts
+  // is set (NB2 <= SEG_STARTS_MAX): the kernel then reads its segment range
+  // from its arguments instead of from memory
+  int seg_starts;
+  int seg_start[SEG_STARTS_MAX + 1];
 };
 
 void gemm_c128(const GemmDesc& d, const c128* A, const c128* B, c128* C, hipStream_t s,
