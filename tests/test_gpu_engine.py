@@ -146,17 +146,20 @@ def check_against_oracle(case):
     eng = make_engine([a.copy() for a in As], Ws, ALG_CODE[alg], chi, dt, eps)
     pops, bds = gpu_trajectory(eng, steps)
     assert np.array_equal(bds, ref["bond_dims"].astype(int))
-    # Tolerance 1e-10, or ten times the trajectory's own conditioning when that
-    # is larger: the oracle is rerun with a 1e-15 relative perturbation of one
-    # input tensor.  (1TDVP with fixed truncated bonds amplifies it ~10x per
-    # step on some rules; from a bare product state it even picks its padding
-    # subspace from rounding-level signs.)
+    # Tolerance per measured step: 1e-10, or ten times the trajectory's own
+    # conditioning at that step when that is larger.  The conditioning is the
+    # change of the oracle's populations when one input tensor is perturbed by
+    # 1e-15 (relative).  1TDVP with fixed truncated bonds amplifies it ~10x per
+    # step on some rules, so only late steps of those runs widen; every run's
+    # first steps are held to 1e-10.
     As2 = [a.copy() for a in As]
     As2[n // 2 - 1] = As2[n // 2 - 1] * (1 + 1e-15)
     o2 = OT.TDVPOracle(As2, Ws, alg, dt, chi, eps, gauge="consistent", expm="lanczos")
-    cond = np.abs(OT.run_trajectory(o2, steps, 1)["population"] - ref["population"]).max()
-    tol = max(1e-10, 10 * cond)
-    assert np.abs(pops - ref["population"]).max() < tol
+    cond = np.abs(OT.run_trajectory(o2, steps, 1)["population"] - ref["population"]).max(axis=1)
+    tol = np.maximum(1e-10, 10 * cond)
+    assert tol.max() <= 1e-6, f"oracle trajectory too ill-conditioned to compare: {tol.max():.1e}"
+    err = np.abs(pops - ref["population"]).max(axis=1)
+    assert (err < tol).all(), f"per-step errors {err} against tolerances {tol}"
 
 
 def test_engine_first_steps_match_reference_golden(gpu):
