@@ -356,7 +356,7 @@ def serve_bench(a):
     import serve
     arrivals = serve.poisson_arrivals(a.serve, a.serve_window, seed=5, big_fraction=a.serve_big_fraction,
                                       max_steps=a.serve_max_steps)
-    log(f"served window ({a.serve_method}): {len(arrivals)} requests over {a.serve_window:.0f} s")
+    log(f"served window ({a.serve_method}): {len(arrivals)} requests over {a.serve_window:g} s")
     cap = a.serve_chi_cap or None
     factory = serve.gpu_engine_factory(precision=1, chi_cap=cap)
     if a.serve_method == "group":
