@@ -491,6 +491,8 @@ GemmPlan gemm_plan(const GemmDesc& d, long ws_elems) {
     QCA_CHECK(d.NB1 == 1 && d.seg_rowptr && d.seg_col && d.seg_outv && d.seg_val && !d.conj_a,
               "gathered GEMM needs one batch level, the sparse operand and no conjugation of A");
     QCA_CHECK(d.seg_max >= 0 && d.K == d.seg_max * d.K2, "gathered GEMM: K must be seg_max * K2");
+    QCA_CHECK(!d.seg_starts || d.NB2 <= SEG_STARTS_MAX,
+              "gathered GEMM: segment starts in the descriptor cover at most SEG_STARTS_MAX batches");
   }
   // per-thread element offsets within a tile are 32-bit and non-negative
   QCA_CHECK(d.a_m >= 0 && d.a_k2 >= 0 && d.b_n >= 0 && d.b_k2 >= 0 && d.a_k1 >= 0 && d.b_k1 >= 0,
