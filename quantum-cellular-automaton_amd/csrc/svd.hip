@@ -19,7 +19,8 @@
 // tol*|x_i||x_j|, and the singular values (final column norms) carry the
 // accuracy of the one-sided method: working precision relative to the largest
 // singular value, which the 1e-10 truncation tail test needs.  A sweep without
-// rotations sets a device flag that turns the remaining launches into no-ops.
+// rotations above the convergence margin (jac_conv_tol) sets a device flag that
+// turns the remaining launches into no-ops.
 // A pair whose smaller column is below 16 eps of the matrix norm is left alone
 // (JAC_NOISE2): such columns are rounding noise of the rotations, their
 // contribution to any tail is < 4e-15, and rotating them against the signal
@@ -102,7 +103,7 @@ __device__ __forceinline__ int rr_player(int t, int r, int n) {
 
 struct JacobiCtrl {
   double fro;                  // ||X||_F^2
-  int done;                    // set by the first sweep without rotations
+  int done;                    // set by the first sweep without a rotation above jac_conv_tol
   int sweeps;                  // sweeps executed
   int rot[JAC_MAX_SWEEPS];     // rotations per sweep
 };
