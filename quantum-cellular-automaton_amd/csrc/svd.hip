@@ -105,7 +105,7 @@ struct JacobiCtrl {
   double fro;                  // ||X||_F^2
   int done;                    // set by the first sweep without a rotation above jac_conv_tol
   int sweeps;                  // sweeps executed
-  int rot[JAC_MAX_SWEEPS];     // rotations per sweep
+  int rot[JAC_MAX_SWEEPS];     // rotations above jac_conv_tol per sweep
 };
 
 struct JacobiGeom {
